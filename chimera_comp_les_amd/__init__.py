@@ -1,0 +1,248 @@
+"""MI355X (gfx950) right-hand-side and time-advance path of Chimera_Comp_LES.
+
+The product is the C ABI in include/cles.h, implemented by libcles.so (HIP
+kernels for gfx950).  This module is a thin ctypes binding of that ABI that
+mirrors the reference's call chain for tests and measurement:
+
+    Solver(cfg, coor)            MG_Initial + FormMetrics      (init.c, metrics.c:3)
+    .set_state(Q)                tail of SetInitialCondition   (rhs.c:424-433)
+    .form_rhs(istage)            formRHS                       (rhs.c:4703)
+    .runge_kutta()               RungeKutta                    (solvers.c:223)
+    .les_update()                Compute_Smagorinsky_Constant_1 + Compute_eddy_viscosity_LES
+                                                               (les.c:135, les.c:1477)
+    .flow_step() / .advance(n)   Flow_Solver + Qold copy       (solvers.c:13, main.c:900-974)
+
+There is no CPU fallback: constructing a Solver without libcles.so or without
+a HIP device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcles.so")
+
+LAYOUT_GLOBAL = 0
+LAYOUT_LOCAL = 1
+PREC_FP64 = 0
+PREC_FP32BULK = 1
+
+ERRORS = {0: "ok", -1: "invalid argument", -2: "out of device memory", -3: "HIP runtime error",
+          -4: "call out of sequence"}
+
+
+class CParams(C.Structure):
+    """Mirror of struct cles_params (include/cles.h)."""
+    _fields_ = [
+        ("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int),
+        ("bctype", C.c_int * 6),
+        ("ren", C.c_double), ("Ma", C.c_double), ("Pr", C.c_double),
+        ("gamma", C.c_double), ("dt", C.c_double), ("st", C.c_double),
+        ("Tref", C.c_double), ("max_cs", C.c_double), ("Twall", C.c_double),
+        ("s_x", C.c_double), ("s_y", C.c_double), ("s_z", C.c_double),
+        ("s_x_min", C.c_double), ("s_y_min", C.c_double), ("s_z_min", C.c_double),
+        ("sigma_thrsh", C.c_double), ("ratio", C.c_double),
+        ("les", C.c_int), ("invicid", C.c_int), ("fluxorder", C.c_int),
+        ("eno_order", C.c_int), ("central", C.c_int), ("sutherland", C.c_int),
+        ("isothermal", C.c_int), ("calc_sigma", C.c_int), ("abc", C.c_int),
+        ("twoD", C.c_int), ("testfilter_ik", C.c_int),
+        ("tistart", C.c_int), ("rstart", C.c_int), ("precision", C.c_int),
+    ]
+
+
+EXPORTS = [
+    "cles_default_params", "cles_create", "cles_destroy", "cles_set_nvert", "cles_set_state",
+    "cles_get_state", "cles_get_metrics", "cles_get_les", "cles_eq_of_state", "cles_form_rhs",
+    "cles_runge_kutta", "cles_les_update", "cles_flow_step", "cles_advance", "cles_norms",
+    "cles_get_time", "cles_set_time", "cles_time_stage_kernel", "cles_time_les",
+    "cles_device_bytes", "cles_last_error",
+]
+
+_LIB = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libcles.so and declare its signatures (no device is touched)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError("libcles.so is not built: run `python -m chimera_comp_les_amd.build`")
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    L.cles_default_params.argtypes = [C.POINTER(CParams), C.c_int, C.c_int, C.c_int]
+    L.cles_default_params.restype = None
+    L.cles_create.argtypes = [C.POINTER(CParams), dp, C.POINTER(C.c_int)]
+    L.cles_create.restype = vp
+    L.cles_destroy.argtypes = [vp]
+    L.cles_destroy.restype = None
+    L.cles_set_nvert.argtypes = [vp, dp]
+    L.cles_set_state.argtypes = [vp, dp, C.c_int]
+    L.cles_get_state.argtypes = [vp, dp, dp, C.c_int]
+    L.cles_get_metrics.argtypes = [vp, dp, dp, dp, dp]
+    L.cles_get_les.argtypes = [vp, dp, dp, dp]
+    L.cles_eq_of_state.argtypes = [vp]
+    L.cles_form_rhs.argtypes = [vp, C.c_int, dp]
+    L.cles_runge_kutta.argtypes = [vp]
+    L.cles_les_update.argtypes = [vp]
+    L.cles_flow_step.argtypes = [vp]
+    L.cles_advance.argtypes = [vp, C.c_int]
+    L.cles_norms.argtypes = [vp, dp]
+    L.cles_get_time.argtypes = [vp]
+    L.cles_set_time.argtypes = [vp, C.c_int]
+    L.cles_time_stage_kernel.argtypes = [vp, C.c_int, dp]
+    L.cles_time_les.argtypes = [vp, C.c_int, dp]
+    L.cles_device_bytes.argtypes = [vp]
+    L.cles_device_bytes.restype = C.c_size_t
+    L.cles_last_error.argtypes = [vp]
+    L.cles_last_error.restype = C.c_char_p
+    _LIB = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_params(cfg):
+    L = load_library()
+    p = CParams()
+    L.cles_default_params(C.byref(p), int(cfg["mx"]), int(cfg["my"]), int(cfg["mz"]))
+    for n, b in enumerate(cfg.get("bctype", (7,) * 6)):
+        p.bctype[n] = int(b)
+    for name, ctype in CParams._fields_:
+        if name in ("mx", "my", "mz", "bctype") or name not in cfg:
+            continue
+        setattr(p, name, float(cfg[name]) if ctype is C.c_double else int(cfg[name]))
+    return p
+
+
+class CLESError(RuntimeError):
+    pass
+
+
+class Solver:
+    """One grid block resident on the current HIP device."""
+
+    def __init__(self, cfg, coor, nvert=None, precision=None):
+        self.L = load_library()
+        cfg = dict(cfg)
+        if precision is not None:
+            cfg["precision"] = precision
+        self.cfg = cfg
+        self.shape = (cfg["mz"], cfg["my"], cfg["mx"])
+        self.p = make_params(cfg)
+        coor = np.ascontiguousarray(coor, dtype=np.float64)
+        if coor.shape != self.shape + (3,):
+            raise ValueError("coordinates must have shape %s" % (self.shape + (3,),))
+        err = C.c_int(0)
+        self.h = self.L.cles_create(C.byref(self.p), _ptr(coor), C.byref(err))
+        if not self.h:
+            raise CLESError("cles_create failed (%s): %s" % (ERRORS.get(err.value, err.value),
+                                                             self.L.cles_last_error(None).decode()))
+        if nvert is not None:
+            nv = np.ascontiguousarray(nvert, dtype=np.float64)
+            self._check(self.L.cles_set_nvert(self.h, _ptr(nv)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CLESError("%s: %s" % (ERRORS.get(rc, rc), self.L.cles_last_error(self.h).decode()))
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cles_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # state
+    def set_state(self, Q, layout=LAYOUT_GLOBAL):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        self._check(self.L.cles_set_state(self.h, _ptr(Q), layout))
+
+    def state(self, layout=LAYOUT_GLOBAL):
+        if layout == LAYOUT_GLOBAL:
+            shp = self.shape
+        else:
+            g = [2 if self.cfg["bctype"][2 * d] == 7 else 0 for d in range(3)]
+            shp = (self.shape[0] + 2 * g[2], self.shape[1] + 2 * g[1], self.shape[2] + 2 * g[0])
+        Q = np.empty(shp + (5,))
+        P = np.empty(shp)
+        self._check(self.L.cles_get_state(self.h, _ptr(Q), _ptr(P), layout))
+        return Q, P
+
+    def metrics(self):
+        csi = np.empty(self.shape + (3,))
+        eta = np.empty(self.shape + (3,))
+        zet = np.empty(self.shape + (3,))
+        aj = np.empty(self.shape)
+        self._check(self.L.cles_get_metrics(self.h, _ptr(csi), _ptr(eta), _ptr(zet), _ptr(aj)))
+        return csi, eta, zet, aj
+
+    def les_fields(self):
+        cs = np.empty(self.shape)
+        nut = np.empty(self.shape)
+        sig = np.empty(self.shape + (3,))
+        self._check(self.L.cles_get_les(self.h, _ptr(cs), _ptr(nut), _ptr(sig)))
+        return cs, nut, sig
+
+    # reference call chain
+    def eq_of_state(self):
+        self._check(self.L.cles_eq_of_state(self.h))
+
+    def form_rhs(self, istage=0):
+        r = np.empty(self.shape + (5,))
+        self._check(self.L.cles_form_rhs(self.h, int(istage), _ptr(r)))
+        return r
+
+    def runge_kutta(self):
+        self._check(self.L.cles_runge_kutta(self.h))
+
+    def les_update(self):
+        self._check(self.L.cles_les_update(self.h))
+
+    def flow_step(self):
+        self._check(self.L.cles_flow_step(self.h))
+
+    def advance(self, n):
+        self._check(self.L.cles_advance(self.h, int(n)))
+
+    def norms(self):
+        out = np.empty(2)
+        self._check(self.L.cles_norms(self.h, _ptr(out)))
+        return out
+
+    @property
+    def ti(self):
+        return self.L.cles_get_time(self.h)
+
+    @ti.setter
+    def ti(self, v):
+        self._check(self.L.cles_set_time(self.h, int(v)))
+
+    # measurement
+    def time_stage_kernel(self, nrep=10):
+        ms = C.c_double(0)
+        self._check(self.L.cles_time_stage_kernel(self.h, int(nrep), C.byref(ms)))
+        return ms.value
+
+    def time_les(self, nrep=3):
+        ms = C.c_double(0)
+        self._check(self.L.cles_time_les(self.h, int(nrep), C.byref(ms)))
+        return ms.value
+
+    def device_bytes(self):
+        return int(self.L.cles_device_bytes(self.h))

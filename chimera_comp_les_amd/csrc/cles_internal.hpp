@@ -1,0 +1,191 @@
+// cles_internal.hpp -- device data structures and shared device functions of
+// the gfx950 right-hand-side / time-advance path.
+//
+// Device layout (MI355X-first, differs from the host DMDA layout on purpose):
+//   every field is stored as structure-of-arrays planes over the mx*my*mz
+//   owned nodes, i fastest, each plane padded to a multiple of 64 nodes
+//   (512 B) so that consecutive lanes read consecutive 8-byte words.
+//   There are no ghost layers: a read at index -2,-1,m,m+1 of a periodic
+//   direction wraps to (idx+m)%m, which is exactly the value the reference's
+//   DMGlobalToLocal puts into its ghost layer (single-rank DMDA, width 2).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/cles.h"
+
+namespace cles {
+
+constexpr int kMaxStages = 3;
+
+struct Grid {
+  int mx, my, mz;
+  int px, py, pz;  // periodic flags per direction (bctype==7, init.c:158-197)
+  long long sy;    // mx
+  long long sz;    // mx*my
+  long long np;    // plane stride (padded node count)
+};
+
+__host__ __device__ __forceinline__ int wrapc(int i, int m, int per) {
+  if (per) {
+    if (i < 0) i += m;
+    else if (i >= m) i -= m;
+  }
+  return i;
+}
+__host__ __device__ __forceinline__ long long nid(const Grid &g, int k, int j, int i) {
+  return (long long)k * g.sz + (long long)j * g.sy + i;
+}
+__host__ __device__ __forceinline__ long long nidw(const Grid &g, int k, int j, int i) {
+  return nid(g, wrapc(k, g.mz, g.pz), wrapc(j, g.my, g.py), wrapc(i, g.mx, g.px));
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction primitives, in the operation order of rhs.c:2256-2568.
+// T is the flux arithmetic type (double for the FP64 drop-in, float for the
+// FP32-bulk form).
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T pmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ __forceinline__ T pmax(T a, T b) { return a < b ? b : a; }
+
+template <typename T> __device__ __forceinline__ T minmod(T a, T b) {
+  if (a * b < T(0)) return T(0);
+  return (a >= T(0) ? T(1) : T(-1)) * pmin(fabs(a), fabs(b));
+}
+template <typename T> __device__ __forceinline__ T muscl_ql(T qm, T q0, T qp) {
+  T S = minmod(qp - q0, q0 - qm);
+  return q0 + T(0.5) * S;
+}
+template <typename T> __device__ __forceinline__ T muscl_qr(T qm, T q0, T qp) {
+  T S = minmod(qp - q0, q0 - qm);
+  return q0 - T(0.5) * S;
+}
+template <typename T> __device__ __forceinline__ T weno_ql(T qm, T q0, T qp) {
+  const T d0 = T(0.6666667), d1 = T(0.3333333), eps = T(1e-6);
+  T Qr0 = T(0.5) * q0 + T(0.5) * qp;
+  T Qr1 = T(-0.5) * qm + T(1.5) * q0;
+  T beta0 = (qp - q0) * (qp - q0);
+  T beta1 = (qm - q0) * (qm - q0);
+  T alfa0 = d0 / ((eps + beta0) * (eps + beta0));
+  T alfa1 = d1 / ((eps + beta1) * (eps + beta1));
+  T alfasum = alfa0 + alfa1;
+  T w0 = alfa0 / alfasum, w1 = alfa1 / alfasum;
+  return w0 * Qr0 + w1 * Qr1;
+}
+template <typename T> __device__ __forceinline__ T weno_qr(T qm, T q0, T qp) {
+  const T d0 = T(0.6666667), d1 = T(0.3333333), eps = T(1e-6);
+  T Qr0 = T(1.5) * q0 - T(0.5) * qp;
+  T Qr1 = T(0.5) * qm + T(0.5) * q0;
+  T beta0 = (qp - q0) * (qp - q0);
+  T beta1 = (qm - q0) * (qm - q0);
+  T alfa0 = d0 / ((eps + beta0) * (eps + beta0));
+  T alfa1 = d1 / ((eps + beta1) * (eps + beta1));
+  T alfasum = alfa0 + alfa1;
+  T w0 = alfa0 / alfasum, w1 = alfa1 / alfasum;
+  return w0 * Qr0 + w1 * Qr1;
+}
+template <typename T> __device__ __forceinline__ T eno_qr(T qm, T q0, T qp, int k) {
+  if (k == 2) return T(0.5) * q0 + T(0.5) * qp;
+  return -qm * T(0.1666666666666666) + q0 * T(0.8333333333333333) + qp * T(0.3333333333333333);
+}
+template <typename T> __device__ __forceinline__ T eno_ql(T qm, T q0, T qp, int k) {
+  if (k == 2) return T(0.5) * q0 + T(0.5) * qm;
+  return qm * T(0.3333333333333333) + q0 * T(0.8333333333333333) - qp * T(0.1666666666666666);
+}
+// skew-symmetric 4-point product, rhs.c:2468-2474
+template <typename T> __device__ __forceinline__ T fp_2(const T u[4], const T v[4]) {
+  return (u[1] + u[2]) * (v[1] + v[2]) / T(3.0) -
+         T(1. / 24.) * (u[0] * v[0] + u[0] * v[2] + u[1] * v[1] + u[1] * v[3] + u[2] * v[2] + u[2] * v[0] +
+                        u[3] * v[1] + u[3] * v[3]);
+}
+template <typename T> __device__ __forceinline__ T fp_1(const T u[4], const T v[4]) {
+  return T(1.0 / 12.0) * (-u[0] * v[0] + T(7) * u[1] * v[1] + T(7) * u[2] * v[2] - u[3] * v[3]);
+}
+template <typename T> __device__ __forceinline__ T fp4(const T u[4]) {
+  return T(1.0 / 12.0) * (-u[0] + T(7) * u[1] + T(7) * u[2] - u[3]);
+}
+
+// Rusanov flux, gamma hard-coded to 1.4 as in rhs.c:2596 / 3256.
+template <typename T>
+__device__ __forceinline__ void rusanov(const T *L, const T *R, const T *h, T uconL, T uconR, T *F) {
+  const T gamma = T(1.4);
+  T gii = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+  T CsoundL = gamma * L[5] / L[0];
+  T CsoundR = gamma * R[5] / R[0];
+  T saL = fabs(uconL) + sqrt(fabs(CsoundL) * gii);
+  T saR = fabs(uconR) + sqrt(fabs(CsoundR) * gii);
+  T SA = pmax(saL, saR);
+  F[0] = T(0.5) * (L[0] * uconL + R[0] * uconR) - T(0.5) * SA * (R[0] - L[0]);
+  F[1] = T(0.5) * ((L[1] * uconL + R[1] * uconR) + (L[5] * h[0] + R[5] * h[0])) - T(0.5) * SA * (R[1] - L[1]);
+  F[2] = T(0.5) * ((L[2] * uconL + R[2] * uconR) + (L[5] * h[1] + R[5] * h[1])) - T(0.5) * SA * (R[2] - L[2]);
+  F[3] = T(0.5) * ((L[3] * uconL + R[3] * uconR) + (L[5] * h[2] + R[5] * h[2])) - T(0.5) * SA * (R[3] - L[3]);
+  F[4] = T(0.5) * ((L[5] + L[4]) * uconL + (R[5] + R[4]) * uconR) - T(0.5) * SA * (R[4] - L[4]);
+}
+template <typename T> __device__ __forceinline__ T ucon_of(const T *s, const T *h) {
+  return h[0] * (s[1] / s[0]) + h[1] * (s[2] / s[0]) + h[2] * (s[3] / s[0]);
+}
+
+// Analytic hybrid-flux weight when -calc_sigma is 0 (rhs.c:3328-3444):
+// Sigma = (s_x, s_y, s_z) on every node, then on interior nodes the
+// curvature test never fires on a uniform field and sigma_y = sigma_z =
+// min(s_z, s_y).
+struct SigmaSpec {
+  double bnd[3];
+  double in[3];
+};
+
+// ---------------------------------------------------------------------------
+// Host-side context.
+// ---------------------------------------------------------------------------
+}  // namespace cles
+
+struct cles_ctx {
+  cles_params p;
+  cles::Grid g;
+  int dev;
+  hipStream_t stream;
+  // static geometry
+  double *coor;             // 3 planes
+  double *csi, *eta, *zet;  // 3 planes each
+  double *aj, *nvert;       // 1 plane each
+  int has_ib;               // any nvert != 0
+  // state: three conserved-variable buffers with their pressures
+  double *Q[3], *P[3];
+  int cur;       // buffer holding Q^n (= Qold)
+  int ucat_src;  // buffer whose velocity the reference's lUcat holds (-1: none yet)
+  int ucat_les_mixed;  // the LES model has overwritten interior lUcat since the last RHS
+  double *rhs;         // 5 planes, written by cles_form_rhs
+  double *xfer;        // 5 planes (+ghosts): host<->device layout staging
+  size_t xfer_elems;
+  // LES
+  double *cs, *cs_alt, *nut;
+  double *grad;    // 10 planes: du_dx..dw_dz, |S|
+  double *lm, *mm;
+  cles::SigmaSpec sig;
+  // reductions
+  unsigned long long *dnorm;  // [2] bit patterns of non-negative doubles
+  double host_norm[2];
+  int ti;
+  int have_state;
+  size_t bytes;
+  char err[256];
+};
+
+namespace cles {
+// launchers implemented in the .hip translation units
+int launch_metrics(cles_ctx *c);
+int launch_eos_all(cles_ctx *c, int buf);
+int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const double coef[3]);
+int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const double coef[3]);
+int launch_les(cles_ctx *c);
+int set_error(cles_ctx *c, int code, const char *fmt, ...);
+}  // namespace cles
+
+#define CLES_HIP(c, call)                                                                      \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return cles::set_error((c), CLES_EDEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #call,      \
+                             hipGetErrorString(e_));                                          \
+  } while (0)

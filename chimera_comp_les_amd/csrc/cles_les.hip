@@ -1,0 +1,379 @@
+// cles_les.hip -- dynamic Smagorinsky model (Compute_Smagorinsky_Constant_1,
+// les.c:135-1472) and eddy viscosity (Compute_eddy_viscosity_LES,
+// les.c:1477-1661), once per time step before the Runge-Kutta stages.
+//
+// Velocity semantics.  The reference keeps a persistent lUcat written by
+// CalcUCat at every RHS evaluation; the Smagorinsky gradient loop then
+// overwrites it node by node in k,j,i order, so at node (k,j,i) its -1
+// neighbours that are interior and not solid already hold the current
+// velocity while its +1 neighbours and all boundary/solid nodes still hold
+// the velocity of the last CalcUCat (the input of the previous step's last
+// stage).  The device keeps that stage-input state resident (one of the
+// three RK buffers) and evaluates both velocities from it, so no velocity
+// field is ever stored.
+//
+// Kernels: gradients -> 27-point test filter (L_ij M_ij contraction) ->
+// 27-point <LM>/<MM> + clipping -> volume-weighted 27-point smoothing ->
+// nu_t.  Each is one thread per node with i-contiguous loads.
+#include "cles_internal.hpp"
+
+namespace cles {
+
+namespace {
+
+constexpr double LES_EPS = 0.000001;  // les.c:8
+
+struct LesArgs {
+  Grid g;
+  const double *q;   // current state (lQ)
+  const double *qs;  // state of the last CalcUCat, or null (lUcat still zero)
+  const double *csi, *eta, *zet, *aj, *nv;
+  int has_ib;
+  double *grad;  // 10 planes: du_dx du_dy du_dz dv_dx dv_dy dv_dz dw_dx dw_dy dw_dz |S|
+  double *lm, *mm;
+  double *cs, *cs_out, *nut;
+  int les, testfilter_ik, ucat_updated;
+  double max_cs;
+};
+
+__device__ __forceinline__ double nvat(const LesArgs &a, long long n) { return a.has_ib ? a.nv[n] : 0.0; }
+__device__ __forceinline__ bool interior(const Grid &g, int k, int j, int i) {
+  return i >= 1 && i <= g.mx - 2 && j >= 1 && j <= g.my - 2 && k >= 1 && k <= g.mz - 2;
+}
+// node visited (and overwritten) by the gradient loop of les.c:250-258
+__device__ __forceinline__ bool in_loop(const LesArgs &a, int k, int j, int i) {
+  return interior(a.g, k, j, i) && !(nvat(a, nid(a.g, k, j, i)) > 1.1);
+}
+__device__ __forceinline__ void vel(const double *q, long long n, long long np, double u[3]) {
+  const double r = q[n];
+  u[0] = q[n + np] / r;
+  u[1] = q[n + 2 * np] / r;
+  u[2] = q[n + 3 * np] / r;
+}
+__device__ __forceinline__ void u_stale(const LesArgs &a, long long n, double u[3]) {
+  if (a.qs) vel(a.qs, n, a.g.np, u);
+  else u[0] = u[1] = u[2] = 0.0;
+}
+// lUcat after the gradient loop has finished
+__device__ __forceinline__ void u_after(const LesArgs &a, int k, int j, int i, double u[3]) {
+  const long long n = nid(a.g, k, j, i);
+  if (a.ucat_updated && in_loop(a, k, j, i)) vel(a.q, n, a.g.np, u);
+  else u_stale(a, n, u);
+}
+
+// Compute_du_center + Compute_du_dxyz (les.c:31-63)
+__device__ __forceinline__ void grad9(const LesArgs &a, long long n, const double up_i[3], const double um_i[3],
+                                      const double up_j[3], const double um_j[3], const double up_k[3],
+                                      const double um_k[3], double gr[9]) {
+  const long long np = a.g.np;
+  double dudc = (up_i[0] - um_i[0]) * 0.5, dvdc = (up_i[1] - um_i[1]) * 0.5, dwdc = (up_i[2] - um_i[2]) * 0.5;
+  double dudz = (up_k[0] - um_k[0]) * 0.5, dvdz = (up_k[1] - um_k[1]) * 0.5, dwdz = (up_k[2] - um_k[2]) * 0.5;
+  double dude = (up_j[0] - um_j[0]) * 0.5, dvde = (up_j[1] - um_j[1]) * 0.5, dwde = (up_j[2] - um_j[2]) * 0.5;
+  const double csi0 = a.csi[n], csi1 = a.csi[n + np], csi2 = a.csi[n + 2 * np];
+  const double eta0 = a.eta[n], eta1 = a.eta[n + np], eta2 = a.eta[n + 2 * np];
+  const double zet0 = a.zet[n], zet1 = a.zet[n + np], zet2 = a.zet[n + 2 * np];
+  const double ajc = a.aj[n];
+  gr[0] = (dudc * csi0 + dude * eta0 + dudz * zet0) * ajc;
+  gr[1] = (dudc * csi1 + dude * eta1 + dudz * zet1) * ajc;
+  gr[2] = (dudc * csi2 + dude * eta2 + dudz * zet2) * ajc;
+  gr[3] = (dvdc * csi0 + dvde * eta0 + dvdz * zet0) * ajc;
+  gr[4] = (dvdc * csi1 + dvde * eta1 + dvdz * zet1) * ajc;
+  gr[5] = (dvdc * csi2 + dvde * eta2 + dvdz * zet2) * ajc;
+  gr[6] = (dwdc * csi0 + dwde * eta0 + dwdz * zet0) * ajc;
+  gr[7] = (dwdc * csi1 + dwde * eta1 + dwdz * zet1) * ajc;
+  gr[8] = (dwdc * csi2 + dwde * eta2 + dwdz * zet2) * ajc;
+}
+__device__ __forceinline__ double sabs_of(const double g[9]) {
+  double Sdd = 2. / 3. * (g[0] + g[4] + g[8]);
+  double Sxx = 0.5 * (g[0] + g[0]) - Sdd, Sxy = 0.5 * (g[1] + g[3]), Sxz = 0.5 * (g[2] + g[6]);
+  double Syx = Sxy, Syy = 0.5 * (g[4] + g[4]) - Sdd, Syz = 0.5 * (g[5] + g[7]);
+  double Szx = Sxz, Szy = Syz, Szz = 0.5 * (g[8] + g[8]) - Sdd;
+  return sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz + Szx * Szx + Szy * Szy +
+                     Szz * Szz));
+}
+
+__global__ void les_grad_kernel(LesArgs a) {
+  const Grid &g = a.g;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long long)g.mx * g.my * g.mz) return;
+  const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
+  const long long np = g.np;
+  if (!in_loop(a, k, j, i)) {
+#pragma unroll
+    for (int c = 0; c < 10; c++) a.grad[n + c * np] = 0.0;
+    return;
+  }
+  double up_i[3], up_j[3], up_k[3], um_i[3], um_j[3], um_k[3];
+  u_stale(a, n + 1, up_i);
+  u_stale(a, n + g.sy, up_j);
+  u_stale(a, n + g.sz, up_k);
+  if (in_loop(a, k, j, i - 1)) vel(a.q, n - 1, np, um_i); else u_stale(a, n - 1, um_i);
+  if (in_loop(a, k, j - 1, i)) vel(a.q, n - g.sy, np, um_j); else u_stale(a, n - g.sy, um_j);
+  if (in_loop(a, k - 1, j, i)) vel(a.q, n - g.sz, np, um_k); else u_stale(a, n - g.sz, um_k);
+  double gr[9];
+  grad9(a, n, up_i, um_i, up_j, um_j, up_k, um_k, gr);
+#pragma unroll
+  for (int c = 0; c < 9; c++) a.grad[n + c * np] = gr[c];
+  a.grad[n + 9 * np] = sabs_of(gr);
+}
+
+// gradient component as seen by the test filter: the local vectors lSx..lSz
+// hold the periodic face copies of les.c:316-377 and zeros on every other
+// boundary node
+__device__ __forceinline__ long long seam_image(const Grid &g, int k, int j, int i) {
+  const bool bi = i == 0 || i == g.mx - 1, bj = j == 0 || j == g.my - 1, bk = k == 0 || k == g.mz - 1;
+  if (bi && !bj && !bk && g.px) return nid(g, k, j, i == 0 ? g.mx - 2 : 1);
+  if (bj && !bi && !bk && g.py) return nid(g, k, j == 0 ? g.my - 2 : 1, i);
+  if (bk && !bi && !bj && g.pz) return nid(g, k == 0 ? g.mz - 2 : 1, j, i);
+  return -1;
+}
+
+__global__ void les_filter_kernel(LesArgs a) {
+  const Grid &g = a.g;
+  const long long np = g.np;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nxi = g.mx - 2, nyi = g.my - 2;
+  if (t >= (long long)nxi * nyi * (g.mz - 2)) return;
+  const int i = 1 + (int)(t % nxi), j = 1 + (int)((t / nxi) % nyi), k = 1 + (int)(t / ((long long)nxi * nyi));
+  const long long n0 = nid(g, k, j, i);
+  if (nvat(a, n0) > 1.1) {
+    a.lm[n0] = 0;
+    a.mm[n0] = 0;
+    return;
+  }
+  // running sums of the 27-point weighted means (integrate_testfilter,
+  // les.c:80-131): summation order [k][j][i]
+  double s_rho = 0, s_ru = 0, s_rv = 0, s_rw = 0, s_ruu = 0, s_ruv = 0, s_ruw = 0, s_rvv = 0, s_rvw = 0, s_rww = 0;
+  double sS[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sSS[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double wsum = 0;
+  double Sx[9];
+  for (int R = 0; R < 3; R++)
+    for (int Qd = 0; Qd < 3; Qd++)
+      for (int Pd = 0; Pd < 3; Pd++) {
+        const int kk = k + R - 1, jj = j + Qd - 1, ii = i + Pd - 1;
+        const long long n = nid(g, kk, jj, ii);
+        const double w = 1. / (fabs(R - 1.0) + 1) * 1. / (fabs(Qd - 1.0) + 1) * 1. / (fabs(Pd - 1.0) + 1);
+        double u[3];
+        u_after(a, kk, jj, ii, u);
+        const double rho = a.q[n], ru = a.q[n + np], rv = a.q[n + 2 * np], rw = a.q[n + 3 * np];
+        double G[9], S;
+        if (interior(g, kk, jj, ii)) {
+#pragma unroll
+          for (int c = 0; c < 9; c++) G[c] = a.grad[n + c * np];
+          S = a.grad[n + 9 * np];
+        } else {
+          const long long s = seam_image(g, kk, jj, ii);
+          if (s >= 0) {
+#pragma unroll
+            for (int c = 0; c < 9; c++) G[c] = a.grad[s + c * np];
+          } else {
+#pragma unroll
+            for (int c = 0; c < 9; c++) G[c] = 0.0;
+          }
+          S = 0.0;  // |S| has no periodic copy (lS, les.c:225-330)
+        }
+        const double Sxx = 0.5 * (G[0] + G[0]), Sxy = 0.5 * (G[1] + G[3]), Sxz = 0.5 * (G[2] + G[6]);
+        const double Syy = 0.5 * (G[4] + G[4]), Syz = 0.5 * (G[5] + G[7]), Szz = 0.5 * (G[8] + G[8]);
+        const double Sv[9] = {Sxx, Sxy, Sxz, Sxy, Syy, Syz, Sxz, Syz, Szz};
+        if (R == 1 && Qd == 1 && Pd == 1) {
+#pragma unroll
+          for (int c = 0; c < 9; c++) Sx[c] = Sv[c];
+        }
+        s_rho += w * rho;
+        s_ru += w * ru;
+        s_rv += w * rv;
+        s_rw += w * rw;
+        s_ruu += w * (ru * u[0]);
+        s_ruv += w * (ru * u[1]);
+        s_ruw += w * (ru * u[2]);
+        s_rvv += w * (rv * u[1]);
+        s_rvw += w * (rv * u[2]);
+        s_rww += w * (rw * u[2]);
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+          sS[c] += w * Sv[c];
+          sSS[c] += w * (Sv[c] * S * rho);
+        }
+        wsum += w;
+      }
+  const double filter = pow(1. / a.aj[n0], 1. / 3.);
+  const double test_filter = a.testfilter_ik ? pow(5.0, 1. / 3.) * filter : 2.0 * filter;
+  const double _rho = s_rho / wsum, _rhou = s_ru / wsum, _rhov = s_rv / wsum, _rhow = s_rw / wsum;
+  double Lij[9];
+  Lij[0] = s_ruu / wsum - _rhou * _rhou / _rho;
+  Lij[1] = s_ruv / wsum - _rhou * _rhov / _rho;
+  Lij[2] = s_ruw / wsum - _rhou * _rhow / _rho;
+  Lij[3] = s_ruv / wsum - _rhov * _rhou / _rho;
+  Lij[4] = s_rvv / wsum - _rhov * _rhov / _rho;
+  Lij[5] = s_rvw / wsum - _rhov * _rhow / _rho;
+  Lij[6] = s_ruw / wsum - _rhow * _rhou / _rho;
+  Lij[7] = s_rvw / wsum - _rhow * _rhov / _rho;
+  Lij[8] = s_rww / wsum - _rhow * _rhow / _rho;
+  double Sh[9], SSh[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    Sh[c] = sS[c] / wsum;
+    SSh[c] = sSS[c] / wsum;
+  }
+  double S_hat = 0;
+#pragma unroll
+  for (int c = 0; c < 9; c++) S_hat += pow(Sh[c], 2.);
+  S_hat = sqrt(2 * S_hat);
+  double L_cat = 0.0, M_cat = 0.0;
+  const double tf2 = pow(test_filter, 2.), f2 = pow(filter, 2.);
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const int ab = p * 3 + q, aa = p * 4, bb = q * 4;
+      M_cat += -2.0 * _rho * tf2 * S_hat * (Sh[ab] * Sx[ab] - 0.33 * Sh[aa] * Sx[bb]) +
+               2.0 * f2 * (SSh[ab] * Sx[ab] - 0.33 * SSh[aa] * Sx[bb]);
+      L_cat += Lij[ab] * Sx[ab] - 0.33 * Sx[aa] * Lij[bb];
+    }
+  a.lm[n0] = L_cat;
+  a.mm[n0] = M_cat;
+}
+
+// <LM>/<MM> and clipping on every owned node (les.c:889-930, 1146-1161)
+__global__ void les_cs_kernel(LesArgs a) {
+  const Grid &g = a.g;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long long)g.mx * g.my * g.mz) return;
+  const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
+  const double nv = nvat(a, n);
+  double cs = a.cs[n];
+  if (interior(g, k, j, i)) {
+    if (nv > 1.1) {
+      cs = 0;
+    } else {
+      double LM_avg = 0.0, MM_avg = 0.0;
+      if (a.les == 3) {
+        LM_avg = a.lm[n];
+        MM_avg = a.mm[n];
+      } else {
+        for (int c = -1; c <= 1; c++)
+          for (int b = -1; b <= 1; b++)
+            for (int aa = -1; aa <= 1; aa++) {
+              const long long m = nid(g, k + c, j + b, i + aa);
+              LM_avg += a.lm[m] / 27.0;
+              MM_avg += a.mm[m] / 27.0;
+            }
+      }
+      const double C = LM_avg / (MM_avg + LES_EPS);
+      cs = a.les == 1 ? 0.01 : (C < 0.0001 ? 0.0001 : C);
+    }
+  }
+  if (nv > 1.1 && nv < 7.) {
+    cs = 0.0;
+  } else {
+    if (nv > 0.1 && nv < 1.1) cs = 0.01 < cs ? cs : 0.01;
+    double t = cs < 0.000 ? 0.000 : cs;
+    cs = t < a.max_cs ? t : a.max_cs;
+  }
+  a.cs[n] = cs;
+}
+
+// value of the local vector lCS at an owned node: periodic face copies
+// (les.c:1196-1256), everything else the global CS
+__device__ __forceinline__ double lcs_at(const LesArgs &a, int k, int j, int i) {
+  const long long s = seam_image(a.g, k, j, i);
+  return a.cs[s >= 0 ? s : nid(a.g, k, j, i)];
+}
+
+// volume-weighted smoothing of Cs over 27 nodes (les.c:1258-1337)
+__global__ void les_smooth_kernel(LesArgs a) {
+  const Grid &g = a.g;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long long)g.mx * g.my * g.mz) return;
+  const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
+  if (!interior(g, k, j, i) || !(nvat(a, n) < 0.1) || a.les == 3) {
+    a.cs_out[n] = a.cs[n];
+    return;
+  }
+  double vs = 0.0, ws = 0.0;
+  for (int c = -1; c <= 1; c++)
+    for (int b = -1; b <= 1; b++)
+      for (int aa = -1; aa <= 1; aa++) {
+        const double w = 1. / a.aj[nid(g, k + c, j + b, i + aa)];
+        vs += w * lcs_at(a, k + c, j + b, i + aa);
+        ws += w;
+      }
+  a.cs_out[n] = vs / ws;
+}
+
+// nu_t = Cs Delta^2 |S| (les.c:1536-1573)
+__global__ void les_nut_kernel(LesArgs a) {
+  const Grid &g = a.g;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long long)g.mx * g.my * g.mz) return;
+  const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
+  if (!interior(g, k, j, i) || nvat(a, n) > 0.1) {
+    a.nut[n] = 0.0;
+    return;
+  }
+  double up_i[3], up_j[3], up_k[3], um_i[3], um_j[3], um_k[3];
+  u_after(a, k, j, i + 1, up_i);
+  u_after(a, k, j + 1, i, up_j);
+  u_after(a, k + 1, j, i, up_k);
+  u_after(a, k, j, i - 1, um_i);
+  u_after(a, k, j - 1, i, um_j);
+  u_after(a, k - 1, j, i, um_k);
+  double gr[9];
+  grad9(a, n, up_i, um_i, up_j, um_j, up_k, um_k, gr);
+  const double Sabs = sabs_of(gr);
+  const double filter = pow(1. / a.aj[n], 1. / 3.);
+  a.nut[n] = a.cs[n] * pow(filter, 2.0) * Sabs;
+}
+
+__global__ void fill_kernel(double *x, long long n, double v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = v;
+}
+
+}  // namespace
+
+int launch_les(cles_ctx *c) {
+  const cles_params &p = c->p;
+  LesArgs a;
+  a.g = c->g;
+  a.q = c->Q[c->cur];
+  a.qs = c->ucat_src >= 0 ? c->Q[c->ucat_src] : nullptr;
+  a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert;
+  a.has_ib = c->has_ib;
+  a.grad = c->grad; a.lm = c->lm; a.mm = c->mm;
+  a.cs = c->cs; a.cs_out = c->cs_alt; a.nut = c->nut;
+  a.les = p.les; a.testfilter_ik = p.testfilter_ik; a.max_cs = p.max_cs;
+  const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
+  const long long NI = (long long)(c->g.mx - 2) * (c->g.my - 2) * (c->g.mz - 2);
+  const int bs = 256;
+  const unsigned gN = (unsigned)((N + bs - 1) / bs), gI = (unsigned)((NI + bs - 1) / bs);
+  if (c->ucat_les_mixed)
+    return set_error(c, CLES_ESTATE, "LES model called twice without a right-hand-side evaluation in between");
+  const bool early = (c->ti < 2 && p.tistart == 0 && !p.rstart) || p.les == 10;
+  a.ucat_updated = 0;
+  if (early || p.les == 1) {
+    // les.c:139-151: Cs = 0 (first step of a fresh run) or 0.03 everywhere
+    hipLaunchKernelGGL(fill_kernel, dim3(gN), dim3(bs), 0, c->stream, c->cs, N, early ? 0.0 : 0.03);
+    CLES_HIP(c, hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(les_grad_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+    a.ucat_updated = 1;
+    hipLaunchKernelGGL(les_filter_kernel, dim3(gI), dim3(bs), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(les_cs_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(les_smooth_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+    double *t = c->cs;
+    c->cs = c->cs_alt;
+    c->cs_alt = t;
+    a.cs = c->cs;
+    c->ucat_les_mixed = 1;
+  }
+  hipLaunchKernelGGL(les_nut_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
+  CLES_HIP(c, hipGetLastError());
+  return CLES_OK;
+}
+
+}  // namespace cles

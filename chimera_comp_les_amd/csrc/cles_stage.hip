@@ -1,0 +1,656 @@
+// cles_stage.hip -- the fused right-hand-side + SSP-RK stage kernel.
+//
+// One launch per RK stage evaluates, for every interior node of the grid,
+//   Conv = sum_d F_d(n) - F_d(n-1)        formConvection / formConvectionLES
+//   Visc = sum_d G_d(n) - G_d(n-1) (+LES)  formViscous
+//   Rhs  = aj (Visc - Conv) or 0           formRHS + MultAj
+// and then either stores Rhs (parity mode) or applies the stage update of
+// RungeKutta (solvers.c:256-271) and the equation of state (rhs.c:4853-4865)
+// in registers, so the only HBM traffic is the stage input, the metrics and
+// the stage output.
+//
+// Work decomposition: a workgroup of 256 threads owns a 32x4x2 tile of
+// interior nodes.  Phase A computes the convective fluxes of the
+// 33*4*2 + 32*5*2 + 32*4*3 = 968 faces touching the tile into LDS, phase B
+// forms the convective residual per node.  Phase C stages the primitive
+// variables (u, v, w, p/rho) of the tile plus a one-node halo in LDS and
+// computes the viscous face fluxes into the same LDS face buffer, phase D
+// forms the viscous residual and the stage update.  Faces are computed once
+// per workgroup (1.26 faces per node-face on average) instead of twice per
+// node.  Tiles are numbered so that each XCD (blockIdx % 8) receives a
+// contiguous slab of tiles and keeps its neighbours in its own L2.
+#include "cles_internal.hpp"
+
+namespace cles {
+
+namespace {
+
+constexpr int TX = 32, TY = 4, TZ = 2, NT = TX * TY * TZ;
+constexpr int NFI = (TX + 1) * TY * TZ;
+constexpr int NFJ = TX * (TY + 1) * TZ;
+constexpr int NFK = TX * TY * (TZ + 1);
+constexpr int NF = NFI + NFJ + NFK;
+constexpr int WX = TX + 2, WY = TY + 2, WZ = TZ + 2, NW = WX * WY * WZ;
+constexpr int NSLOT = 64;  // spread of the inf-norm atomics
+
+struct StageArgs {
+  Grid g;
+  const double *q, *p;  // stage input (5 planes, 1 plane)
+  const double *qold;   // Qold (5 planes)
+  double *qo, *po;      // stage output
+  double *rhs;          // parity mode output (5 planes)
+  const double *csi, *eta, *zet, *aj, *nv, *nut;
+  int bc[6];
+  int fluxorder, eno, central, iso, has_ib;
+  double ren, Ma, Pr, gamma, Tref, ratio, dt;
+  int suth;
+  SigmaSpec sig;
+  double a_rhs, a_old, a_cur;
+  int istage;
+  int ntx, nty, ntz, per_xcd;
+  unsigned long long *norms;  // [2][NSLOT]
+};
+
+template <typename T>
+__device__ __forceinline__ void load6(const StageArgs &a, long long n, T s[6]) {
+  const long long np = a.g.np;
+  s[0] = T(a.q[n]);
+  s[1] = T(a.q[n + np]);
+  s[2] = T(a.q[n + 2 * np]);
+  s[3] = T(a.q[n + 3 * np]);
+  s[4] = T(a.q[n + 4 * np]);
+  s[5] = T(a.p[n]);
+}
+__device__ __forceinline__ double nvat(const StageArgs &a, long long n) { return a.has_ib ? a.nv[n] : 0.0; }
+__device__ __forceinline__ const double *metric(const StageArgs &a, int d) {
+  return d == 0 ? a.csi : d == 1 ? a.eta : a.zet;
+}
+__device__ __forceinline__ bool interior(const Grid &g, int k, int j, int i) {
+  return i >= 1 && i <= g.mx - 2 && j >= 1 && j <= g.my - 2 && k >= 1 && k <= g.mz - 2;
+}
+__device__ __forceinline__ double sigma_at(const StageArgs &a, int d, int k, int j, int i) {
+  return interior(a.g, k, j, i) ? a.sig.in[d] : a.sig.bnd[d];
+}
+
+template <typename T>
+__device__ __forceinline__ void recon_muscl(const StageArgs &a, long long nm, long long n0, long long n1,
+                                            long long n2, T *L, T *R) {
+  T s0[6], s1[6], sm[6], sp[6];
+  load6(a, n0, s0);
+  load6(a, n1, s1);
+  if (nvat(a, nm) > 2.9) load6(a, n1, sm); else load6(a, nm, sm);
+  if (nvat(a, n2) > 2.9) load6(a, n0, sp); else load6(a, n2, sp);
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    L[q] = muscl_ql(sm[q], s0[q], s1[q]);
+    R[q] = muscl_qr(s0[q], s1[q], sp[q]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void recon_weno(const StageArgs &a, long long nm, long long n0, long long n1,
+                                           long long n2, bool eno_left, int eno_mask, T *L, T *R) {
+  T sm[6], s0[6], s1[6], s2[6];
+  load6(a, nm, sm);
+  load6(a, n0, s0);
+  load6(a, n1, s1);
+  load6(a, n2, s2);
+  const int order = a.eno;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    if (eno_left || (eno_mask >> q & 1))
+      L[q] = eno_ql(s0[q], s1[q], s2[q], order);
+    else
+      L[q] = weno_ql(sm[q], s0[q], s1[q]);
+  }
+  if (nvat(a, n2) > 2.9) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) R[q] = eno_qr(sm[q], s0[q], s1[q], order);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 6; q++) R[q] = weno_qr(s0[q], s1[q], s2[q]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void face_metric(const StageArgs &a, int d, long long n0, long long n1, T h[3]) {
+  const double *m = metric(a, d);
+  const long long np = a.g.np;
+  h[0] = T(0.5 * (m[n0] + m[n1]));
+  h[1] = T(0.5 * (m[n0 + np] + m[n1 + np]));
+  h[2] = T(0.5 * (m[n0 + 2 * np] + m[n1 + 2 * np]));
+}
+
+// formConvection face (rhs.c:2646-2814, 2817-2971, 2973-3121)
+template <typename T>
+__device__ void conv_face_plain(const StageArgs &a, int d, int k, int j, int i, T F[5]) {
+  const Grid &g = a.g;
+  const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
+  const int id = d == 0 ? i : d == 1 ? j : k;
+  const long long s = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
+  const long long n0 = nid(g, k, j, i), n1 = n0 + s;
+  const bool iso = a.iso != 0;
+  T L[6], R[6];
+  if (d == 1 && j == 0 && iso && a.bc[2] == 1) {
+    load6(a, n0, L);
+    load6(a, n0, R);
+  } else if (d == 1 && j == g.my - 2 && iso && a.bc[3] == 1) {
+    load6(a, n1, L);
+    load6(a, n1, R);
+  } else if (a.fluxorder == 1 || id == 0 || id == m - 2 || (nvat(a, n0) + nvat(a, n1)) > 2.9) {
+    load6(a, n0, L);
+    load6(a, n1, R);
+  } else {
+    const long long nm = n0 - s, n2 = n0 + 2 * s;
+    if (a.fluxorder == 2 || (d != 0 && j < 30 && k < 7)) {
+      recon_muscl(a, nm, n0, n1, n2, L, R);
+    } else {
+      bool eno_left = nvat(a, nm) > 2.9;
+      int mask = 0;
+      if (d == 0) eno_left = eno_left && (i + 2) < g.mx;
+      if (d == 1 && !eno_left && j == 1 && iso && a.bc[2] == 1) mask = (1 << 0) | (1 << 4) | (1 << 5);
+      recon_weno(a, nm, n0, n1, n2, eno_left, mask, L, R);
+    }
+  }
+  T h[3];
+  face_metric(a, d, n0, n1, h);
+  T uconL = (id == m - 2 && a.bc[2 * d + 1] == 1) ? ucon_of(R, h) : ucon_of(L, h);
+  T uconR = (id == 0 && a.bc[2 * d] == 1) ? uconL : ucon_of(R, h);
+  rusanov(L, R, h, uconL, uconR, F);
+}
+
+// formConvectionLES face (rhs.c:3466-3706, 3714-3937, 3947-4168)
+template <typename T>
+__device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, T F[5]) {
+  const Grid &g = a.g;
+  const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
+  const int id = d == 0 ? i : d == 1 ? j : k;
+  const long long s = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
+  const long long n0 = nid(g, k, j, i), n1 = n0 + s;
+  const long long np = g.np;
+  T h[3];
+  face_metric(a, d, n0, n1, h);
+  T Fc[5];
+  bool four = false;
+  if (id > 0 && id < m - 2 && (a.central == 2 || a.central == 3)) {
+    const long long nm = n0 - s, n2 = n0 + 2 * s;
+    double sum = d == 0 ? nvat(a, n0) + nvat(a, n1) + nvat(a, n2) + nvat(a, nm)
+                        : nvat(a, nm) + nvat(a, n0) + nvat(a, n1) + nvat(a, n2);
+    if (sum < 2.9) {
+      four = true;
+      const double *mt = metric(a, d);
+      T r[4], u[4], v[4], w[4], ep[4], pp[4], U[4];
+      const long long nn[4] = {nm, n0, n1, n2};
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const long long n = nn[q];
+        r[q] = T(a.q[n]);
+        u[q] = T(a.q[n + np]);
+        v[q] = T(a.q[n + 2 * np]);
+        w[q] = T(a.q[n + 3 * np]);
+        T e = T(a.q[n + 4 * np]);
+        pp[q] = T(a.p[n]);
+        ep[q] = d == 2 ? e + pp[q] : pp[q] + e;
+        U[q] = T(1.) / r[q] * (T(mt[n]) * u[q] + T(mt[n + np]) * v[q] + T(mt[n + 2 * np]) * w[q]);
+      }
+      if (a.central == 2) {
+        Fc[0] = fp_2(r, U);
+        Fc[1] = fp_2(u, U) + fp4(pp) * h[0];
+        Fc[2] = fp_2(v, U) + fp4(pp) * h[1];
+        Fc[3] = fp_2(w, U) + fp4(pp) * h[2];
+        Fc[4] = fp_2(ep, U);
+      } else {
+        Fc[0] = fp_1(r, U);
+        Fc[1] = fp_1(u, U) + fp4(pp) * h[0];
+        Fc[2] = fp_1(v, U) + fp4(pp) * h[1];
+        Fc[3] = fp_1(w, U) + fp4(pp) * h[2];
+        Fc[4] = fp_1(ep, U);
+      }
+    }
+  }
+  if (!four) {
+    T s0[6], s1[6];
+    load6(a, n0, s0);
+    load6(a, n1, s1);
+    T rc = T(0.5) * (s0[0] + s1[0]);
+    T uc = T(0.5) * (s0[1] + s1[1]);
+    T vc = T(0.5) * (s0[2] + s1[2]);
+    T wc = T(0.5) * (s0[3] + s1[3]);
+    T ec = T(0.5) * (s0[4] + s1[4]);
+    T Uc = (h[0] * uc + h[1] * vc + h[2] * wc) / rc;
+    T pc = T(0.5) * (s0[5] + s1[5]);
+    Fc[0] = rc * Uc;
+    Fc[1] = uc * Uc + pc * h[0];
+    Fc[2] = vc * Uc + pc * h[1];
+    Fc[3] = wc * Uc + pc * h[2];
+    Fc[4] = Uc * (ec + pc);
+  }
+  const T sigma = T(sigma_at(a, d, k, j, i));
+  if (sigma == T(1)) {
+    // (1 - sigma) * F_upwind == 0 for every finite upwind flux: the blend of
+    // rhs.c:3698-3702 reduces to the central flux exactly.
+#pragma unroll
+    for (int q = 0; q < 5; q++) F[q] = Fc[q];
+    return;
+  }
+  // upwind part; periodic faces 0 and m-2 read ghost nodes (wrapped)
+  const long long nm = nidw(g, k - (d == 2), j - (d == 1), i - (d == 0));
+  const long long n2 = nidw(g, k + 2 * (d == 2), j + 2 * (d == 1), i + 2 * (d == 0));
+  const double nvpair = d == 2 ? nvat(a, n0) + nvat(a, nid(g, k + 1, j, 1)) : nvat(a, n0) + nvat(a, n1);
+  T L[6], R[6];
+  if (a.fluxorder == 1 || ((id == 0 || id == m - 2) && a.bc[2 * d] != 7) || nvpair > 2.9) {
+    load6(a, n0, L);
+    load6(a, n1, R);
+  } else if (a.fluxorder == 2) {
+    recon_muscl(a, nm, n0, n1, n2, L, R);
+  } else {
+    bool eno_left = nvat(a, nm) > 2.9;
+    if (d == 0) eno_left = eno_left && (i + 2) < g.mx;
+    recon_weno(a, nm, n0, n1, n2, eno_left, 0, L, R);
+  }
+  T Fw[5];
+  rusanov(L, R, h, ucon_of(L, h), ucon_of(R, h), Fw);
+#pragma unroll
+  for (int q = 0; q < 5; q++) F[q] = Fw[q] * (T(1.0) - sigma) + Fc[q] * sigma;
+}
+
+// ------------------------------ viscous ----------------------------------
+struct Win {  // LDS window of primitives with a one-node halo
+  int i0, j0, k0;  // global coordinates of window node (0,0,0)
+};
+__device__ __forceinline__ int widx(const Win &w, int k, int j, int i) {
+  return ((k - w.k0) * WY + (j - w.j0)) * WX + (i - w.i0);
+}
+__device__ __forceinline__ bool ibn(double nv) { return nv > 0.5 && nv < 7.; }
+
+// one-sided cross derivatives next to immersed-boundary rows (rhs.c:1498-1575)
+template <typename T>
+__device__ __forceinline__ void cross_deriv(const StageArgs &a, const T *W, int wA, int wB, int wt,
+                                            long long gA, long long gB, long long gt, bool q8, T du[3], T &dT) {
+  const T *U = W, *V = W + NW, *Wz = W + 2 * NW, *Tm = W + 3 * NW;
+  int mode = 2;
+  if (a.has_ib) {
+    if (ibn(a.nv[gA + gt]) || ibn(a.nv[gB + gt])) mode = 0;
+    else if (ibn(a.nv[gA - gt]) || ibn(a.nv[gB - gt])) mode = 1;
+  }
+  if (mode == 0) {
+    du[0] = (U[wB] + U[wA] - U[wB - wt] - U[wA - wt]) * T(0.5);
+    du[1] = (V[wB] + V[wA] - V[wB - wt] - V[wA - wt]) * T(0.5);
+    du[2] = (Wz[wB] + Wz[wA] - Wz[wB - wt] - Wz[wA - wt]) * T(0.5);
+    dT = q8 ? (Tm[wB] + Tm[wA] - Tm[wB - wt] - Tm[wA]) * T(0.5)
+            : (Tm[wB] + Tm[wA] - Tm[wB - wt] - Tm[wA - wt]) * T(0.5);
+  } else if (mode == 1) {
+    du[0] = (U[wB + wt] + U[wA + wt] - U[wB] - U[wA]) * T(0.5);
+    du[1] = (V[wB + wt] + V[wA + wt] - V[wB] - V[wA]) * T(0.5);
+    du[2] = (Wz[wB + wt] + Wz[wA + wt] - Wz[wB] - Wz[wA]) * T(0.5);
+    dT = (Tm[wB + wt] + Tm[wA + wt] - Tm[wB] - Tm[wA]) * T(0.5);
+  } else {
+    du[0] = (U[wB + wt] + U[wA + wt] - U[wB - wt] - U[wA - wt]) * T(0.25);
+    du[1] = (V[wB + wt] + V[wA + wt] - V[wB - wt] - V[wA - wt]) * T(0.25);
+    du[2] = (Wz[wB + wt] + Wz[wA + wt] - Wz[wB - wt] - Wz[wA - wt]) * T(0.25);
+    dT = (Tm[wB + wt] + Tm[wA + wt] - Tm[wB - wt] - Tm[wA - wt]) * T(0.25);
+  }
+}
+
+// formViscous face (rhs.c:1487-1707, 1719-1918, 1926-2121); G[0..3] =
+// rhoU..rhoE flux, al = LES stress (alpha) of the face.
+template <typename T, bool LES>
+__device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d, int k, int j, int i, T G[4],
+                          T al[3]) {
+  const Grid &g = a.g;
+  const long long np = g.np;
+  const long long gs[3] = {1, g.sy, g.sz};
+  const int ws[3] = {1, WX, WX * WY};
+  const long long gA = nid(g, k, j, i), gB = gA + gs[d];
+  const int wA = widx(win, k, j, i), wB = wA + ws[d];
+  const T *U = W, *V = W + NW, *Wz = W + 2 * NW, *Tm = W + 3 * NW;
+  T dd[3][3], dT[3];
+  dd[d][0] = U[wB] - U[wA];
+  dd[d][1] = V[wB] - V[wA];
+  dd[d][2] = Wz[wB] - Wz[wA];
+  dT[d] = Tm[wB] - Tm[wA];
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    if (t == d) continue;
+    cross_deriv(a, W, wA, wB, ws[t], gA, gB, gs[t], d == 0 && t == 2, dd[t], dT[t]);
+  }
+  const double *cs = a.csi, *et = a.eta, *ze = a.zet;
+  T csi0 = T(0.5 * (cs[gA] + cs[gB])), csi1 = T(0.5 * (cs[gA + np] + cs[gB + np])),
+    csi2 = T(0.5 * (cs[gA + 2 * np] + cs[gB + 2 * np]));
+  T eta0 = T(0.5 * (et[gA] + et[gB])), eta1 = T(0.5 * (et[gA + np] + et[gB + np])),
+    eta2 = T(0.5 * (et[gA + 2 * np] + et[gB + 2 * np]));
+  T zet0 = T(0.5 * (ze[gA] + ze[gB])), zet1 = T(0.5 * (ze[gA + np] + ze[gB + np])),
+    zet2 = T(0.5 * (ze[gA + 2 * np] + ze[gB + 2 * np]));
+  T m0, m1, m2, g11, g21, g31;
+  if (d == 0) {
+    m0 = csi0; m1 = csi1; m2 = csi2;
+    g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2;
+    g21 = eta0 * csi0 + eta1 * csi1 + eta2 * csi2;
+    g31 = zet0 * csi0 + zet1 * csi1 + zet2 * csi2;
+  } else if (d == 1) {
+    m0 = eta0; m1 = eta1; m2 = eta2;
+    g11 = csi0 * eta0 + csi1 * eta1 + csi2 * eta2;
+    g21 = eta0 * eta0 + eta1 * eta1 + eta2 * eta2;
+    g31 = zet0 * eta0 + zet1 * eta1 + zet2 * eta2;
+  } else {
+    m0 = zet0; m1 = zet1; m2 = zet2;
+    g11 = csi0 * zet0 + csi1 * zet1 + csi2 * zet2;
+    g21 = eta0 * zet0 + eta1 * zet1 + eta2 * zet2;
+    g31 = zet0 * zet0 + zet1 * zet1 + zet2 * zet2;
+  }
+  const T dudc = dd[0][0], dvdc = dd[0][1], dwdc = dd[0][2];
+  const T dude = dd[1][0], dvde = dd[1][1], dwde = dd[1][2];
+  const T dudz = dd[2][0], dvdz = dd[2][1], dwdz = dd[2][2];
+  T r11 = dudc * csi0 + dude * eta0 + dudz * zet0;
+  T r21 = dvdc * csi0 + dvde * eta0 + dvdz * zet0;
+  T r31 = dwdc * csi0 + dwde * eta0 + dwdz * zet0;
+  T r12 = dudc * csi1 + dude * eta1 + dudz * zet1;
+  T r22 = dvdc * csi1 + dvde * eta1 + dvdz * zet1;
+  T r32 = dwdc * csi1 + dwde * eta1 + dwdz * zet1;
+  T r13 = dudc * csi2 + dude * eta2 + dudz * zet2;
+  T r23 = dvdc * csi2 + dvde * eta2 + dvdz * zet2;
+  T r33 = dwdc * csi2 + dwde * eta2 + dwdz * zet2;
+  T ajc = T(0.5 * (a.aj[gA] + a.aj[gB]));
+  const double rA = a.q[gA], rB = a.q[gB];
+  T nu = T(1. / a.ren);
+  if (LES) nu = T(1. / a.ren * 0.5 * (rA + a.q[gA + 1]));  // rho at [i] and [i+1] on every face
+  if (a.suth) {
+    const double C_sutherland = 110.4;
+    double Tr = a.gamma * a.Ma * a.Ma * (a.p[gA] + a.p[gB]) / (rA + rB);
+    if (Tr < 0)
+      nu = T(1. / a.ren);
+    else
+      nu = T(1. / a.ren * (1 + C_sutherland / a.Tref) / (Tr + C_sutherland / a.Tref) * pow(Tr, 1.5));
+  }
+  T div = T(2. / 3.) * (dudc * csi0 + dude * eta0 + dudz * zet0 + dvdc * csi1 + dvde * eta1 + dvdz * zet1 +
+                        dwdc * csi2 + dwde * eta2 + dwdz * zet2);
+  T fu = (g11 * dudc + g21 * dude + g31 * dudz + r11 * m0 + r21 * m1 + r31 * m2) * ajc * (nu);
+  T fv = (g11 * dvdc + g21 * dvde + g31 * dvdz + r12 * m0 + r22 * m1 + r32 * m2) * ajc * (nu);
+  T fw = (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * m0 + r23 * m1 + r33 * m2) * ajc * (nu);
+  if (d == 2) {
+    fu = T(0.0) + fu;
+    fv = T(0.0) + fv;
+    fw = T(0.0) + fw;
+  }
+  fu -= div * m0 * ajc * nu;
+  fv -= div * m1 * ajc * nu;
+  fw -= div * m2 * ajc * nu;
+  const int id = d == 0 ? i : d == 1 ? j : k;
+  const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
+  T uh0, uh1, uh2;
+  if (id == 0 && a.bc[2 * d] == 1) {
+    uh0 = U[wA]; uh1 = V[wA]; uh2 = Wz[wA];
+  } else if (id == m - 2 && a.bc[2 * d + 1] == 1) {
+    uh0 = U[wB]; uh1 = V[wB]; uh2 = Wz[wB];
+  } else {
+    uh0 = T(0.5) * (U[wB] + U[wA]);
+    uh1 = T(0.5) * (V[wB] + V[wA]);
+    uh2 = T(0.5) * (Wz[wB] + Wz[wA]);
+  }
+  const T gam = T(a.gamma), pr = T(a.Pr);
+  T fe = (fu * uh0 + fv * uh1 + fw * uh2) + (g11 * dT[0] + g21 * dT[1] + g31 * dT[2]) * ajc * (gam * nu / pr / (gam - T(1.0)));
+  if (LES) {
+    const T CI = T(0.06);
+    T sA = T(sigma_at(a, d, k, j, i));
+    T sB = T(sigma_at(a, d, k + (d == 2), j + (d == 1), i + (d == 0)));
+    T nu_t = T(0.5) * (T(a.nut[gA]) * sA + T(a.nut[gB]) * sB);
+    T rho_c = T(0.5 * (rA + rB));
+    T sigma_c = T(0.5) * (sA + sB);
+    if ((a.bc[2 * d] == 1 && id == 0) || (a.bc[2 * d + 1] == 1 && id == m - 2)) nu_t = T(0);
+    T tx = (g11 * dudc + g21 * dude + g31 * dudz + r11 * m0 + r21 * m1 + r31 * m2 - div * m0) * ajc * (nu_t * rho_c);
+    T ty = (g11 * dvdc + g21 * dvde + g31 * dvdz + r12 * m0 + r22 * m1 + r32 * m2 - div * m1) * ajc * (nu_t * rho_c);
+    T tz = (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * m0 + r23 * m1 + r33 * m2 - div * m2) * ajc * (nu_t * rho_c);
+    T S2 = T(.5) * ((r11 - div) * (r11 - div) + r12 * r12 + r13 * r13 + r21 * r21 + (r22 - div) * (r22 - div) +
+                    r23 * r23 + r31 * r31 + r32 * r32 + (r33 - div) * (r33 - div));
+    double filter = pow(1. / a.aj[gA], 1. / 3.);
+    T f2 = T(pow(filter, 2.0));
+    T Ssx = -S2 * f2 * CI * T(2.0) / T(3.0) * m0 * rho_c * sigma_c;
+    T Ssy = -S2 * f2 * CI * T(2.0) / T(3.0) * m1 * rho_c * sigma_c;
+    T Ssz = -S2 * f2 * CI * T(2.0) / T(3.0) * m2 * rho_c * sigma_c;
+    fu += tx + Ssx;
+    fv += ty + Ssy;
+    fw += tz + Ssz;
+    al[0] = tx + Ssx;
+    al[1] = ty + Ssy;
+    al[2] = tz + Ssz;
+  }
+  G[0] = fu;
+  G[1] = fv;
+  G[2] = fw;
+  G[3] = fe;
+}
+
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  return v;
+}
+
+// face decode: f in [0, NF) -> direction and local coordinates
+__device__ __forceinline__ void decode_face(int f, int &d, int &lx, int &ly, int &lz) {
+  if (f < NFI) {
+    d = 0; lx = f % (TX + 1); ly = (f / (TX + 1)) % TY; lz = f / ((TX + 1) * TY);
+  } else if (f < NFI + NFJ) {
+    f -= NFI;
+    d = 1; lx = f % TX; ly = (f / TX) % (TY + 1); lz = f / (TX * (TY + 1));
+  } else {
+    f -= NFI + NFJ;
+    d = 2; lx = f % TX; ly = (f / TX) % TY; lz = f / (TX * TY);
+  }
+}
+__device__ __forceinline__ int fi_of(int lx, int ly, int lz) { return (lz * TY + ly) * (TX + 1) + lx; }
+__device__ __forceinline__ int fj_of(int lx, int ly, int lz) { return NFI + (lz * (TY + 1) + ly) * TX + lx; }
+__device__ __forceinline__ int fk_of(int lx, int ly, int lz) { return NFI + NFJ + (lz * TY + ly) * TX + lx; }
+
+template <typename T, int NC>
+struct StageSmem {
+  T F[NC][NF];
+};
+
+template <typename T, bool LES, bool VISC, int MODE>
+__global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
+  constexpr int FC = VISC ? (LES ? 7 : 5) : 5;
+  __shared__ T Fs[FC][NF];
+  __shared__ T Ws[VISC ? 4 * NW : 1];
+  const Grid &g = a.g;
+  // XCD-aware tile numbering: blockIdx % 8 selects the XCD, each XCD walks a
+  // contiguous range of tiles.
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * a.per_xcd + (b >> 3);
+  if (tile >= a.ntx * a.nty * a.ntz) return;
+  const int tix = tile % a.ntx, tiy = (tile / a.ntx) % a.nty, tiz = tile / (a.ntx * a.nty);
+  const int i0 = 1 + tix * TX, j0 = 1 + tiy * TY, k0 = 1 + tiz * TZ;
+  const int t = threadIdx.x;
+  const long long np = g.np;
+
+  // ---- phase A: convective face fluxes -> LDS
+  for (int f = t; f < NF; f += NT) {
+    int d, lx, ly, lz;
+    decode_face(f, d, lx, ly, lz);
+    int gi = i0 + lx - (d == 0), gj = j0 + ly - (d == 1), gk = k0 + lz - (d == 2);
+    bool ok = d == 0 ? (gi <= g.mx - 2 && gj <= g.my - 2 && gk <= g.mz - 2)
+                     : d == 1 ? (gj <= g.my - 2 && gi <= g.mx - 2 && gk <= g.mz - 2)
+                              : (gk <= g.mz - 2 && gi <= g.mx - 2 && gj <= g.my - 2);
+    if (!ok) continue;
+    T F[5];
+    if (LES) conv_face_les<T>(a, d, gk, gj, gi, F);
+    else conv_face_plain<T>(a, d, gk, gj, gi, F);
+#pragma unroll
+    for (int q = 0; q < 5; q++) Fs[q][f] = F[q];
+  }
+  __syncthreads();
+
+  // ---- phase B: convective residual of this thread's node
+  const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
+  const int gi = i0 + lx, gj = j0 + ly, gk = k0 + lz;
+  const bool valid = gi <= g.mx - 2 && gj <= g.my - 2 && gk <= g.mz - 2;
+  const long long n = valid ? nid(g, gk, gj, gi) : 0;
+  double conv[5];
+  {
+    const int a0 = fi_of(lx, ly, lz), a1 = fi_of(lx + 1, ly, lz);
+    const int b0 = fj_of(lx, ly, lz), b1 = fj_of(lx, ly + 1, lz);
+    const int c0 = fk_of(lx, ly, lz), c1 = fk_of(lx, ly, lz + 1);
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+      conv[q] = double(Fs[q][a1] - Fs[q][a0] + Fs[q][b1] - Fs[q][b0] + Fs[q][c1] - Fs[q][c0]);
+    if (LES && a.ratio != 0.0 && valid) {
+      conv[3] = conv[3] - a.ratio / a.aj[n] * a.q[n] / a.dt;
+      conv[4] = conv[4] - a.ratio / a.aj[n] * a.q[n + 3 * np] / a.dt;
+    }
+  }
+  double visc[5] = {0., 0., 0., 0., 0.};
+  if (VISC) {
+    __syncthreads();  // everyone is done reading the convective faces
+    // ---- phase C1: stage u, v, w, p/rho of the tile and a one-node halo
+    Win win{i0 - 1, j0 - 1, k0 - 1};
+    for (int w = t; w < NW; w += NT) {
+      int wx = w % WX, wy = (w / WX) % WY, wz = w / (WX * WY);
+      int ii = win.i0 + wx, jj = win.j0 + wy, kk = win.k0 + wz;
+      if (ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
+      long long m = nid(g, kk, jj, ii);
+      double r = a.q[m];
+      Ws[w] = T(a.q[m + np] / r);
+      Ws[NW + w] = T(a.q[m + 2 * np] / r);
+      Ws[2 * NW + w] = T(a.q[m + 3 * np] / r);
+      Ws[3 * NW + w] = T(a.p[m] / r);
+    }
+    __syncthreads();
+    // ---- phase C2: viscous face fluxes -> LDS
+    for (int f = t; f < NF; f += NT) {
+      int d, fx, fy, fz;
+      decode_face(f, d, fx, fy, fz);
+      int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
+      bool ok = fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2;
+      if (!ok) continue;
+      T G[4], al[3] = {T(0), T(0), T(0)};
+      visc_face<T, LES>(a, Ws, win, d, fk, fj, fi, G, al);
+#pragma unroll
+      for (int q = 0; q < 4; q++) Fs[q][f] = G[q];
+      if (LES) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) Fs[4 + q][f] = al[q];
+      }
+    }
+    __syncthreads();
+    // ---- phase D1: viscous residual (+ LES energy term, rhs.c:2153-2197)
+    const int a0 = fi_of(lx, ly, lz), a1 = fi_of(lx + 1, ly, lz);
+    const int b0 = fj_of(lx, ly, lz), b1 = fj_of(lx, ly + 1, lz);
+    const int c0 = fk_of(lx, ly, lz), c1 = fk_of(lx, ly, lz + 1);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      visc[q + 1] = double(Fs[q][a1] - Fs[q][a0] + Fs[q][b1] - Fs[q][b0] + Fs[q][c1] - Fs[q][c0]);
+    if (LES && valid && gj > 2 && gj < g.my - 2 && !(a.has_ib && !(a.nv[n] < 0.5))) {
+      const double *r = a.q;
+      const long long sy = g.sy, sz = g.sz;
+      T rL = T((r[n] + r[n - 1]) * 0.5), rR = T((r[n + 1] + r[n]) * 0.5);
+      T rD = T((r[n] + r[n - sy]) * 0.5), rU = T((r[n + sy] + r[n]) * 0.5);
+      T rB = T((r[n] + r[n - sz]) * 0.5), rF = T((r[n + sz] + r[n]) * 0.5);
+      T u0 = T(a.q[n + np] / r[n]), u1 = T(a.q[n + 2 * np] / r[n]), u2 = T(a.q[n + 3 * np] / r[n]);
+      T e = u0 * (rR * Fs[4][a1] - rL * Fs[4][a0] + rU * Fs[4][b1] - rD * Fs[4][b0] + rF * Fs[4][c1] - rB * Fs[4][c0]) +
+            u1 * (rR * Fs[5][a1] - rL * Fs[5][a0] + rU * Fs[5][b1] - rD * Fs[5][b0] + rF * Fs[5][c1] - rB * Fs[5][c0]) +
+            u2 * (rR * Fs[6][a1] - rL * Fs[6][a0] + rU * Fs[6][b1] - rD * Fs[6][b0] + rF * Fs[6][c1] - rB * Fs[6][c0]);
+      visc[4] = double(T(visc[4]) + e);
+    }
+  }
+  if (!valid) return;
+
+  // ---- phase D2: Rhs = aj (Visc - Conv) (MultAj, rhs.c:4611-4629)
+  double rhs[5];
+  const bool fluid = !a.has_ib || a.nv[n] < 0.1;
+  const double ajn = a.aj[n];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    double r = visc[q] - conv[q];
+    rhs[q] = fluid ? r * ajn : 0.;
+  }
+  double qin[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) qin[q] = a.q[n + q * np];
+  if (a.norms) {
+    double mr = 0., mq = 0.;
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      mr = fmax(mr, fabs(rhs[q]));
+      mq = fmax(mq, fabs(qin[q]));
+    }
+    mr = wave_max(mr);
+    mq = wave_max(mq);
+    if ((t & 63) == 0) {
+      const int slot = (b * 4 + (t >> 6)) & (NSLOT - 1);
+      atomicMax(&a.norms[slot], dbits(mr));
+      atomicMax(&a.norms[NSLOT + slot], dbits(mq));
+    }
+  }
+  if (MODE == 0) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) a.rhs[n + q * np] = rhs[q];
+    return;
+  }
+  // ---- stage update, VecWAXPY / VecAXPBYPCZ (solvers.c:256-271)
+  double qn[5];
+  if (a.istage == 0) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) qn[q] = a.a_rhs * rhs[q] + a.qold[n + q * np];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; q++) qn[q] = a.a_old * a.qold[n + q * np] + a.a_rhs * rhs[q] + a.a_cur * qin[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 5; q++) a.qo[n + q * np] = qn[q];
+  // EqOfState (rhs.c:4853-4857)
+  a.po[n] = (a.gamma - 1) * (qn[4] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) / qn[0]);
+}
+
+template <typename T, bool LES, bool VISC>
+hipError_t launch_t(const StageArgs &a, int mode, int nblocks, hipStream_t s) {
+  if (mode == 0)
+    hipLaunchKernelGGL((stage_kernel<T, LES, VISC, 0>), dim3(nblocks), dim3(NT), 0, s, a);
+  else
+    hipLaunchKernelGGL((stage_kernel<T, LES, VISC, 1>), dim3(nblocks), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const double coef[3]) {
+  const cles_params &p = c->p;
+  StageArgs a;
+  a.g = c->g;
+  a.q = c->Q[in];
+  a.p = c->P[in];
+  a.qold = c->Q[c->cur];
+  a.qo = out >= 0 ? c->Q[out] : nullptr;
+  a.po = out >= 0 ? c->P[out] : nullptr;
+  a.rhs = c->rhs;
+  a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert; a.nut = c->nut;
+  for (int q = 0; q < 6; q++) a.bc[q] = p.bctype[q];
+  a.fluxorder = p.fluxorder; a.eno = p.eno_order; a.central = p.central; a.iso = p.isothermal;
+  a.has_ib = c->has_ib;
+  a.ren = p.ren; a.Ma = p.Ma; a.Pr = p.Pr; a.gamma = p.gamma; a.Tref = p.Tref; a.ratio = p.ratio; a.dt = p.dt;
+  a.suth = p.sutherland;
+  a.sig = c->sig;
+  a.a_rhs = coef[0]; a.a_old = coef[1]; a.a_cur = coef[2];
+  a.istage = istage;
+  a.ntx = (c->g.mx - 2 + TX - 1) / TX;
+  a.nty = (c->g.my - 2 + TY - 1) / TY;
+  a.ntz = (c->g.mz - 2 + TZ - 1) / TZ;
+  const int ntiles = a.ntx * a.nty * a.ntz;
+  a.per_xcd = (ntiles + 7) / 8;
+  a.norms = c->dnorm;
+  const int nblocks = a.per_xcd * 8;
+  const bool les = p.les != 0, visc = !p.invicid;
+  hipError_t e;
+  if (p.precision == CLES_PREC_FP32BULK) {
+    if (les) e = visc ? launch_t<float, true, true>(a, mode, nblocks, c->stream) : launch_t<float, true, false>(a, mode, nblocks, c->stream);
+    else e = visc ? launch_t<float, false, true>(a, mode, nblocks, c->stream) : launch_t<float, false, false>(a, mode, nblocks, c->stream);
+  } else {
+    if (les) e = visc ? launch_t<double, true, true>(a, mode, nblocks, c->stream) : launch_t<double, true, false>(a, mode, nblocks, c->stream);
+    else e = visc ? launch_t<double, false, true>(a, mode, nblocks, c->stream) : launch_t<double, false, false>(a, mode, nblocks, c->stream);
+  }
+  CLES_HIP(c, e);
+  return CLES_OK;
+}
+
+}  // namespace cles

@@ -1,0 +1,148 @@
+/*
+ * cles.h -- C ABI of the MI355X (gfx950) right-hand-side and time-advance path
+ * for Chimera_Comp_LES.
+ *
+ * The entry points replace the reference's CPU call chain
+ *   Flow_Solver -> RungeKutta -> formRHS / FormBCS / EqOfState
+ *   and Compute_Smagorinsky_Constant_1 / Compute_eddy_viscosity_LES
+ * with one context per grid block whose fields stay resident in HBM.
+ *
+ * Data exchanged with the host uses the reference's PETSc DMDA layout:
+ *   - node order i fastest, then j, then k (DMDAVecGetArray [k][j][i]);
+ *   - conserved variables interleaved {rho, rhoU, rhoV, rhoW, rhoE}
+ *     (CompVars, variables.h:22-24); coordinates and metric triples {x,y,z}
+ *     (Cmpnts, variables.h:18-20);
+ *   - layout CLES_LAYOUT_GLOBAL: the mx*my*mz owned nodes (a global Vec);
+ *     layout CLES_LAYOUT_LOCAL: the ghosted local Vec of a single-rank DMDA
+ *     with stencil width 2 (ghosts only in periodic directions,
+ *     init.c:156-228), so a PETSc host may pass DMDAVecGetArray base pointers.
+ *
+ * Conventions: every function returns 0 on success and a negative CLES_E*
+ * code on failure (the reference returns PetscErrorCode 0); no function
+ * calls exit().  All calls are synchronous on return.  A context is not
+ * thread-safe; use one context per host thread.
+ */
+#ifndef CLES_H
+#define CLES_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLES_OK 0
+#define CLES_EINVAL (-1)     /* bad argument / unsupported option combination */
+#define CLES_ENOMEM (-2)     /* device allocation failed */
+#define CLES_EDEVICE (-3)    /* HIP runtime error */
+#define CLES_ESTATE (-4)     /* call out of sequence (e.g. no state uploaded) */
+
+#define CLES_LAYOUT_GLOBAL 0
+#define CLES_LAYOUT_LOCAL 1
+
+/* Arithmetic of the face reconstruction and flux kernels. */
+#define CLES_PREC_FP64 0       /* plain FP64 drop-in */
+#define CLES_PREC_FP32BULK 1   /* FP32 reconstruction/flux, FP64 metrics, RK accumulation and EOS */
+
+/* Run options.  Names and defaults follow the reference's PETSc options
+ * (init.c:85-96, main.c:40-72, rhs.c:2581-2585, les.c:151). */
+typedef struct cles_params {
+  int mx, my, mz;        /* IM JM KM of grid.dat (nodes incl. boundary)      */
+  int bctype[6];         /* bcs.dat: i-, i+, j-, j+, k-, k+ (rhs.c:456-475)  */
+  double ren;            /* -ren   Reynolds number                            */
+  double Ma;             /* -Ma    Mach number                                */
+  double Pr;             /* -Pr    Prandtl number                             */
+  double gamma;          /* -gamma ratio of specific heats                    */
+  double dt;             /* -dt    time step                                  */
+  double st;             /* user->st (1)                                      */
+  double Tref;           /* -Tref  Sutherland reference temperature           */
+  double max_cs;         /* -max_cs clip of the dynamic Smagorinsky constant  */
+  double Twall;          /* -Twall isothermal wall temperature                */
+  double s_x, s_y, s_z;  /* -s_x -s_y -s_z hybrid-flux weights                */
+  double s_x_min, s_y_min, s_z_min;
+  double sigma_thrsh;    /* -sigma_thrsh                                      */
+  double ratio;          /* channel forcing ratio (0 without -channel)        */
+  int les;               /* -les: 0 off, 1 constant Cs, >=2 dynamic           */
+  int invicid;           /* -inv                                              */
+  int fluxorder;         /* -fluxorder: 1 Godunov, 2 MUSCL(+RK2), 3 WENO(+RK3)*/
+  int eno_order;         /* -ENOorder                                         */
+  int central;           /* -central: 2 skew-symmetric 4-point (default)      */
+  int sutherland;        /* -visT                                             */
+  int isothermal;        /* -isothermal                                       */
+  int calc_sigma;        /* -calc_sigma (must be 0 on this path)              */
+  int abc;               /* -ABC                                              */
+  int twoD;              /* -twoD (must be 0 on this path)                    */
+  int testfilter_ik;     /* -testfilter_ik                                    */
+  int tistart;           /* first time step of the run (0 = fresh start)      */
+  int rstart;            /* restart flag                                      */
+  int precision;         /* CLES_PREC_*                                       */
+} cles_params;
+
+typedef struct cles_ctx cles_ctx;
+
+/* Fill p with the reference defaults for an mx*my*mz block. */
+void cles_default_params(cles_params *p, int mx, int my, int mz);
+
+/* Create a context on the current HIP device, upload the node coordinates
+ * (global layout, 3 per node) and form the metrics on the device.
+ * Replaces MG_Initial's allocation (init.c:956-1138) followed by
+ * FormMetrics (metrics.c:3-782).  *err receives a CLES_E* code. */
+cles_ctx *cles_create(const cles_params *p, const double *coor, int *err);
+void cles_destroy(cles_ctx *c);
+
+/* Immersed-boundary node flags (Nvert, one per node, global layout):
+ * 0 fluid, 1 IB node, 3 solid (ibm_iso.c:762-769).  Default: all fluid. */
+int cles_set_nvert(cles_ctx *c, const double *nvert);
+
+/* Upload the conserved state; sets Q, Qold and the pressure by EqOfState.
+ * Replaces the tail of SetInitialCondition (rhs.c:424-433). */
+int cles_set_state(cles_ctx *c, const double *Q, int layout);
+/* Download Q (5 per node) and/or P (1 per node); either may be NULL. */
+int cles_get_state(cles_ctx *c, double *Q, double *P, int layout);
+/* Metrics as formed by FormMetrics: csi, eta, zet (3 per node), aj. */
+int cles_get_metrics(cles_ctx *c, double *csi, double *eta, double *zet, double *aj);
+/* LES fields: Cs (CS), nu_t (Nu_t), sigma (Sigma, 3 per node); any may be NULL. */
+int cles_get_les(cles_ctx *c, double *cs, double *nu_t, double *sigma);
+
+/* EqOfState, rhs.c:4820: P from Q on every node. */
+int cles_eq_of_state(cles_ctx *c);
+/* formRHS, rhs.c:4703: Rhs = aj (Visc - Conv) on interior fluid nodes, 0
+ * elsewhere; istage selects the sigma refresh of formConvectionLES
+ * (rhs.c:3327).  rhs (5 per node, global layout) may be NULL. */
+int cles_form_rhs(cles_ctx *c, int istage, double *rhs);
+/* RungeKutta, solvers.c:223: fluxorder stages of formRHS, stage update,
+ * FormBCS, EqOfState (no immersed body). */
+int cles_runge_kutta(cles_ctx *c);
+/* Compute_Smagorinsky_Constant_1 (les.c:135) followed by
+ * Compute_eddy_viscosity_LES (les.c:1477). */
+int cles_les_update(cles_ctx *c);
+/* One time step of the reference loop (main.c:900-974 with Flow_Solver,
+ * solvers.c:13-64): ti += 1, LES model, RungeKutta, Qold = Q. */
+int cles_flow_step(cles_ctx *c);
+/* nsteps calls of cles_flow_step without host synchronisation in between. */
+int cles_advance(cles_ctx *c, int nsteps);
+
+/* Inf-norms of the last stage: out[0] = |Rhs|inf, out[1] = |Q|inf
+ * (VecNorm NORM_INFINITY, solvers.c:251-252). */
+int cles_norms(cles_ctx *c, double out[2]);
+int cles_get_time(cles_ctx *c);
+int cles_set_time(cles_ctx *c, int ti);
+
+/* Kernel timing for measurement: runs the fused stage kernel nrep times
+ * back to back on the context's stream (stage-1 update of the current state
+ * into a scratch buffer, so the state is unchanged) between two HIP events
+ * and returns the mean duration per launch in milliseconds. */
+int cles_time_stage_kernel(cles_ctx *c, int nrep, double *ms);
+/* Same for one LES model update (all of its kernels); the LES fields are
+ * recomputed from the current state, nothing else changes. */
+int cles_time_les(cles_ctx *c, int nrep, double *ms);
+
+/* Device bytes held by the context. */
+size_t cles_device_bytes(const cles_ctx *c);
+/* Last error message of the context (static storage when c is NULL). */
+const char *cles_last_error(const cles_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

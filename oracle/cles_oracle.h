@@ -1,0 +1,115 @@
+/*
+ * cles_oracle.h -- CPU restatement of the Chimera_Comp_LES right-hand-side and
+ * time-advance path, used ONLY as the correctness checker and as the CPU
+ * baseline timer.  It is test infrastructure: the product library
+ * (chimera_comp_les_amd/csrc) never includes, links or calls it.
+ *
+ * Layout follows the reference's PETSc DMDA single-rank arrays:
+ *   - "global" arrays hold the mx*my*mz owned nodes, i fastest
+ *     (init.c:202-228 creates da/cda/fda with dims IM,JM,KM);
+ *   - "local" arrays carry 2 ghost layers in every periodic direction
+ *     (stencil width s=2, init.c:156; periodic iff bctype==7, init.c:158-197)
+ *     and no ghosts in non-periodic directions.  They are indexed with global
+ *     indices, ghosts at -2,-1 and m,m+1, filled by a periodic wrap
+ *     (DMGlobalToLocal semantics).
+ *   - conserved variables are interleaved {rho,rhoU,rhoV,rhoW,rhoE}
+ *     (variables.h:22-24), metric triples {x,y,z} (variables.h:18-20).
+ *
+ * Every function cites the reference file:line it restates.  Arithmetic is
+ * written in the reference's operation order and the library is compiled with
+ * -ffp-contract=off so that results are reproducible.
+ *
+ * Parity status of each part is recorded in the tests that pin it
+ * (tests/test_oracle_*.py).  Parts that no closed form, invariant or brute
+ * force pins are marked "parity unpinned" next to their definition.
+ */
+#ifndef CLES_ORACLE_H
+#define CLES_ORACLE_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double rho, rhoU, rhoV, rhoW, rhoE; } orc_q;
+typedef struct { double x, y, z; } orc_v3;
+
+/* Run-time options of the reference (PETSc options DB names in comments). */
+typedef struct {
+  int mx, my, mz;          /* IM JM KM from grid.dat (init.c:690)            */
+  int bctype[6];           /* bcs.dat i-,i+,j-,j+,k-,k+ (init.c:555-559)      */
+  double ren;              /* -ren   (init.c:90, default 3000)               */
+  double Ma;               /* -Ma    (default 2.81)                          */
+  double Pr;               /* -Pr    (default 0.72)                          */
+  double gamma;            /* -gamma (default 1.4)                           */
+  double dt;               /* -dt    (default 0.01)                          */
+  double st;               /* user->st = 1 (init.c:98)                       */
+  double Tref;             /* -Tref  (rhs.c:1415, default 273.15)            */
+  double max_cs;           /* -max_cs (les.c:151-153, default 0.05)          */
+  double Twall;            /* -Twall (rhs.c:511, default 1)                  */
+  double s_x, s_y, s_z;    /* -s_x.. (rhs.c:3311-3313, default 1)            */
+  double s_x_min, s_y_min, s_z_min; /* (rhs.c:3311, default 0)              */
+  double sigma_thrsh;      /* -sigma_thrsh -> ks (rhs.c:3323-3325, 1e-5)     */
+  double ratio;            /* user->ratio, channel forcing (0 unless -channel)*/
+  int les;                 /* -les (main.c:53,772)                           */
+  int invicid;             /* -inv                                           */
+  int fluxorder;           /* -fluxorder (rhs.c:2582, default 3)             */
+  int eno_order;           /* -ENOorder  (rhs.c:2581, default 2)             */
+  int central;             /* -central   (main.c:52, default 2)              */
+  int sutherland;          /* -visT      (rhs.c:1413)                        */
+  int isothermal;          /* -isothermal                                    */
+  int calc_sigma;          /* -calc_sigma (rhs.c:3335)                       */
+  int abc;                 /* -ABC (rhs.c:3307)                              */
+  int twoD;                /* -twoD, read by FormBCS (rhs.c:512)             */
+  int testfilter_ik;       /* -testfilter_ik (main.c:67,788)                 */
+  int tistart;             /* time step the run starts from (main.c:890)     */
+  int rstart;              /* restart flag (main.c:810)                      */
+} orc_params;
+
+typedef struct orc_ctx orc_ctx;
+
+orc_ctx *orc_create(const orc_params *p, const double *coor_global);
+void orc_destroy(orc_ctx *c);
+void orc_set_threads(int n);
+
+/* nvert (global, one per node) -> Nvert/lNvert */
+void orc_set_nvert(orc_ctx *c, const double *nvert);
+/* Q (global interleaved) -> Q, lQ, Qold, lQold, then EqOfState
+ * (tail of SetInitialCondition, rhs.c:424-433). */
+void orc_set_state(orc_ctx *c, const double *Q);
+void orc_get_state(const orc_ctx *c, double *Q, double *P);
+/* metric getters: csi,eta,zet as 3 per node, aj one per node (global) */
+void orc_get_metrics(const orc_ctx *c, double *csi, double *eta, double *zet, double *aj);
+void orc_get_les(const orc_ctx *c, double *cs, double *nu_t, double *sigma);
+void orc_set_time(orc_ctx *c, int ti);
+int  orc_get_time(const orc_ctx *c);
+
+void orc_eq_of_state(orc_ctx *c);                       /* rhs.c:4820 */
+void orc_form_rhs(orc_ctx *c, int istage, double *rhs); /* rhs.c:4703 */
+void orc_form_bcs(orc_ctx *c, double alfa);             /* rhs.c:477  */
+void orc_runge_kutta(orc_ctx *c);                       /* solvers.c:223 */
+void orc_les_update(orc_ctx *c);  /* les.c:135 + les.c:1477 */
+void orc_flow_step(orc_ctx *c);   /* main.c:900-974 one step, no IB */
+void orc_norms(const orc_ctx *c, double out[2]);       /* solvers.c:251-252 */
+
+/* component building blocks exported for known-answer tests */
+double orc_weno_ql(double qm, double q0, double qp);
+double orc_weno_qr(double qm, double q0, double qp);
+double orc_eno_ql(double qm, double q0, double qp, int k);
+double orc_eno_qr(double qm, double q0, double qp, int k);
+double orc_muscl_ql(double qm, double q0, double qp);
+double orc_muscl_qr(double qm, double q0, double qp);
+double orc_minmod(double a, double b);
+double orc_fp_2(const double u[4], const double v[4]);
+double orc_fp4(const double u[4]);
+void orc_rusanov(const double ql[5], double pl, const double qr[5], double pr,
+                 const double csih[3], double out[5]);
+/* Literal in-place Smagorinsky gradient loop, used to pin the two-array
+ * restatement (les.c:250-309). Writes Ax,Ay,Az (9/node) and Sabs. */
+void orc_les_gradients_literal(orc_ctx *c, double *A9, double *sabs);
+void orc_les_gradients(orc_ctx *c, double *A9, double *sabs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,47 @@
+/* Internal definitions of the CPU restatement (test infrastructure only). */
+#ifndef CLES_ORACLE_INTERNAL_H
+#define CLES_ORACLE_INTERNAL_H
+#include "cles_oracle.h"
+#include <math.h>
+#include <stddef.h>
+
+struct orc_ctx {
+  orc_params p;
+  int mx, my, mz;
+  int gx, gy, gz;        /* ghost width per direction: 2 if periodic else 0 */
+  int LX, LY, LZ;        /* local (ghosted) extents */
+  size_t nG, nL;
+  orc_v3 *coor;          /* global coordinates */
+  orc_v3 *lcsi, *leta, *lzet;
+  double *laj;
+  double *nvert, *lnvert;
+  orc_q *Q, *Qold, *Rhs, *lQ;
+  double *P, *lP;
+  orc_v3 *Sigma, *lSigma;
+  double *CS, *lCS, *Nu_t, *lNu_t;
+  orc_v3 *lUcat;
+  int ti;
+  /* per-call temporaries the reference obtains with VecDuplicate */
+  orc_q *fp1, *fp2, *fp3, *conv, *visc;
+  orc_v3 *al1, *al2, *al3;
+};
+
+#define GI(c, k, j, i) ((((size_t)(k)) * (size_t)(c)->my + (size_t)(j)) * (size_t)(c)->mx + (size_t)(i))
+#define LI(c, k, j, i)                                                          \
+  ((((size_t)((k) + (c)->gz)) * (size_t)(c)->LY + (size_t)((j) + (c)->gy)) *   \
+       (size_t)(c)->LX +                                                        \
+   (size_t)((i) + (c)->gx))
+
+void orc_g2l(const orc_ctx *c, const double *g, double *l, int nc);
+void orc_l2l(const orc_ctx *c, double *l, int nc);
+
+/* PetscMin / PetscMax as macros in the reference (petscmath.h) */
+static inline double pmin(double a, double b) { return a < b ? a : b; }
+static inline double pmax(double a, double b) { return a < b ? b : a; }
+
+void orc_form_convection(orc_ctx *c);           /* rhs.c:2570 */
+void orc_form_convection_les(orc_ctx *c, int istage); /* rhs.c:3231 */
+void orc_calc_ucat(orc_ctx *c);                 /* rhs.c:1322 */
+void orc_form_viscous(orc_ctx *c);              /* rhs.c:1367 */
+
+#endif

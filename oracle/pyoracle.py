@@ -1,0 +1,195 @@
+"""ctypes binding of the CPU restatement (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker, never as the thing
+measured or shipped.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int),
+        ("bctype", C.c_int * 6),
+        ("ren", C.c_double), ("Ma", C.c_double), ("Pr", C.c_double),
+        ("gamma", C.c_double), ("dt", C.c_double), ("st", C.c_double),
+        ("Tref", C.c_double), ("max_cs", C.c_double), ("Twall", C.c_double),
+        ("s_x", C.c_double), ("s_y", C.c_double), ("s_z", C.c_double),
+        ("s_x_min", C.c_double), ("s_y_min", C.c_double), ("s_z_min", C.c_double),
+        ("sigma_thrsh", C.c_double), ("ratio", C.c_double),
+        ("les", C.c_int), ("invicid", C.c_int), ("fluxorder", C.c_int),
+        ("eno_order", C.c_int), ("central", C.c_int), ("sutherland", C.c_int),
+        ("isothermal", C.c_int), ("calc_sigma", C.c_int), ("abc", C.c_int),
+        ("twoD", C.c_int), ("testfilter_ik", C.c_int),
+        ("tistart", C.c_int), ("rstart", C.c_int),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(OrcParams), dp]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_threads.argtypes = [C.c_int]
+        for name in ("orc_set_nvert", "orc_set_state"):
+            getattr(L, name).argtypes = [C.c_void_p, dp]
+        L.orc_get_state.argtypes = [C.c_void_p, dp, dp]
+        L.orc_get_metrics.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.orc_get_les.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_set_time.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_time.argtypes = [C.c_void_p]
+        L.orc_get_time.restype = C.c_int
+        L.orc_eq_of_state.argtypes = [C.c_void_p]
+        L.orc_form_rhs.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_form_bcs.argtypes = [C.c_void_p, C.c_double]
+        L.orc_runge_kutta.argtypes = [C.c_void_p]
+        L.orc_les_update.argtypes = [C.c_void_p]
+        L.orc_flow_step.argtypes = [C.c_void_p]
+        L.orc_norms.argtypes = [C.c_void_p, dp]
+        L.orc_les_gradients.argtypes = [C.c_void_p, dp, dp]
+        L.orc_les_gradients_literal.argtypes = [C.c_void_p, dp, dp]
+        for name in ("orc_weno_ql", "orc_weno_qr", "orc_muscl_ql", "orc_muscl_qr"):
+            f = getattr(L, name)
+            f.restype = C.c_double
+            f.argtypes = [C.c_double] * 3
+        for name in ("orc_eno_ql", "orc_eno_qr"):
+            f = getattr(L, name)
+            f.restype = C.c_double
+            f.argtypes = [C.c_double] * 3 + [C.c_int]
+        L.orc_minmod.restype = C.c_double
+        L.orc_minmod.argtypes = [C.c_double, C.c_double]
+        L.orc_fp_2.restype = C.c_double
+        L.orc_fp_2.argtypes = [dp, dp]
+        L.orc_fp4.restype = C.c_double
+        L.orc_fp4.argtypes = [dp]
+        L.orc_rusanov.argtypes = [dp, C.c_double, dp, C.c_double, dp, dp]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_params(cfg):
+    """cfg: dict with the option names of OrcParams (missing -> reference defaults)."""
+    p = OrcParams()
+    defaults = dict(ren=3000.0, Ma=2.81, Pr=0.72, gamma=1.4, dt=0.01, st=1.0, Tref=273.15,
+                    max_cs=0.05, Twall=1.0, s_x=1.0, s_y=1.0, s_z=1.0, s_x_min=0.0,
+                    s_y_min=0.0, s_z_min=0.0, sigma_thrsh=1e-5, ratio=0.0, les=0, invicid=0,
+                    fluxorder=3, eno_order=2, central=2, sutherland=0, isothermal=0,
+                    calc_sigma=0, abc=0, twoD=0, testfilter_ik=0, tistart=0, rstart=0)
+    d = dict(defaults)
+    d.update({k: v for k, v in cfg.items() if k in defaults})
+    p.mx, p.my, p.mz = cfg["mx"], cfg["my"], cfg["mz"]
+    for n, b in enumerate(cfg["bctype"]):
+        p.bctype[n] = int(b)
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+class Oracle:
+    """Owns one orc_ctx.  Arrays use the global DMDA layout (k, j, i, comp)."""
+
+    def __init__(self, cfg, coor, nvert=None, threads=None):
+        self.L = lib()
+        if threads:
+            self.L.orc_set_threads(int(threads))
+        self.cfg = cfg
+        self.shape = (cfg["mz"], cfg["my"], cfg["mx"])
+        self.p = make_params(cfg)
+        coor = np.ascontiguousarray(coor, dtype=np.float64)
+        assert coor.shape == self.shape + (3,)
+        self.h = self.L.orc_create(C.byref(self.p), _ptr(coor))
+        if nvert is not None:
+            nv = np.ascontiguousarray(nvert, dtype=np.float64)
+            assert nv.shape == self.shape
+            self.L.orc_set_nvert(self.h, _ptr(nv))
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        assert Q.shape == self.shape + (5,)
+        self.L.orc_set_state(self.h, _ptr(Q))
+
+    def state(self):
+        Q = np.empty(self.shape + (5,))
+        P = np.empty(self.shape)
+        self.L.orc_get_state(self.h, _ptr(Q), _ptr(P))
+        return Q, P
+
+    def metrics(self):
+        csi = np.empty(self.shape + (3,))
+        eta = np.empty(self.shape + (3,))
+        zet = np.empty(self.shape + (3,))
+        aj = np.empty(self.shape)
+        self.L.orc_get_metrics(self.h, _ptr(csi), _ptr(eta), _ptr(zet), _ptr(aj))
+        return csi, eta, zet, aj
+
+    def les_fields(self):
+        cs = np.empty(self.shape)
+        nut = np.empty(self.shape)
+        sig = np.empty(self.shape + (3,))
+        self.L.orc_get_les(self.h, _ptr(cs), _ptr(nut), _ptr(sig))
+        return cs, nut, sig
+
+    def form_rhs(self, istage=0):
+        r = np.empty(self.shape + (5,))
+        self.L.orc_form_rhs(self.h, int(istage), _ptr(r))
+        return r
+
+    def eq_of_state(self):
+        self.L.orc_eq_of_state(self.h)
+
+    def runge_kutta(self):
+        self.L.orc_runge_kutta(self.h)
+
+    def les_update(self):
+        self.L.orc_les_update(self.h)
+
+    def step(self, n=1):
+        for _ in range(n):
+            self.L.orc_flow_step(self.h)
+
+    @property
+    def ti(self):
+        return self.L.orc_get_time(self.h)
+
+    @ti.setter
+    def ti(self, v):
+        self.L.orc_set_time(self.h, int(v))
+
+    def norms(self):
+        out = np.empty(2)
+        self.L.orc_norms(self.h, _ptr(out))
+        return out
+
+    def les_gradients(self, literal=False):
+        A = np.empty(self.shape + (9,))
+        S = np.empty(self.shape)
+        f = self.L.orc_les_gradients_literal if literal else self.L.orc_les_gradients
+        f(self.h, _ptr(A), _ptr(S))
+        return A, S

@@ -1,0 +1,48 @@
+"""The C ABI library loads and exports every entry point include/cles.h declares
+(no device calls: this runs on the CPU)."""
+import ctypes
+import os
+import re
+
+import chimera_comp_les_amd as cl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "cles.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(cles_[a-z_]+)\s*\(", src)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(cl.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    L = cl.load_library()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+        assert ctypes.cast(getattr(L, name), ctypes.c_void_p).value
+
+
+def test_params_struct_layout():
+    # field order/size of cles_params as the header declares it
+    src = open(os.path.join(ROOT, "include", "cles.h")).read()
+    body = src[src.index("typedef struct cles_params {"):src.index("} cles_params;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = []
+    for decl in body.split(";")[:-1]:
+        decl = decl.split("{")[-1].strip()
+        typ, rest = decl.split(None, 1)
+        for n in rest.split(","):
+            names.append(n.strip().split("[")[0])
+    assert names == [f[0] for f in cl.CParams._fields_]
+
+
+def test_default_params_match_reference_defaults():
+    p = cl.make_params(dict(mx=8, my=9, mz=10))
+    assert (p.mx, p.my, p.mz) == (8, 9, 10)
+    assert p.ren == 3000.0 and p.Ma == 2.81 and p.Pr == 0.72 and p.gamma == 1.4   # init.c:85-96
+    assert p.dt == 0.01 and p.fluxorder == 3 and p.eno_order == 2 and p.central == 2
+    assert p.max_cs == 0.05 and p.Tref == 273.15 and p.sigma_thrsh == 1e-5
