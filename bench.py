@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--scheme", choices=["weno", "muscl"], default="weno")
     ap.add_argument("--precision", choices=["fp64", "fp32bulk"], default="fp64")
     ap.add_argument("--case", choices=["tgv", "vortex"], default="tgv")
-    ap.add_argument("--cpu-N", type=int, default=48)
+    ap.add_argument("--cpu-N", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
