@@ -108,6 +108,9 @@ void orc_rusanov(const double ql[5], double pl, const double qr[5], double pr,
  * restatement (les.c:250-309). Writes Ax,Ay,Az (9/node) and Sabs. */
 void orc_les_gradients_literal(orc_ctx *c, double *A9, double *sabs);
 void orc_les_gradients(orc_ctx *c, double *A9, double *sabs);
+/* Register caller-owned arrays (global layout) that receive LM and MM at
+ * every later dynamic-model update (NULL to stop). */
+void orc_les_debug(orc_ctx *c, double *lm, double *mm);
 
 #ifdef __cplusplus
 }

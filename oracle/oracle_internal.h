@@ -24,6 +24,7 @@ struct orc_ctx {
   /* per-call temporaries the reference obtains with VecDuplicate */
   orc_q *fp1, *fp2, *fp3, *conv, *visc;
   orc_v3 *al1, *al2, *al3;
+  double *dbg_lm, *dbg_mm; /* caller-owned copies of LM/MM for tests */
 };
 
 #define GI(c, k, j, i) ((((size_t)(k)) * (size_t)(c)->my + (size_t)(j)) * (size_t)(c)->mx + (size_t)(i))

@@ -271,6 +271,14 @@ static void smagorinsky(orc_ctx *c) {
       }
   orc_l2l(c, LM, 1);
   orc_l2l(c, MM, 1);
+  if (c->dbg_lm) {
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+          c->dbg_lm[GI(c, k, j, i)] = LM[LI(c, k, j, i)];
+          c->dbg_mm[GI(c, k, j, i)] = MM[LI(c, k, j, i)];
+        }
+  }
 
   /* Cs = <LM>/(<MM>+eps) with 27-point means (les.c:889-930) */
   for (int k = 1; k < mz - 1; k++)
@@ -353,6 +361,11 @@ static void eddy_viscosity(orc_ctx *c) {
         c->Nu_t[g] = c->CS[g] * pow(filter, 2.0) * Sabs;
       }
   orc_g2l(c, c->Nu_t, c->lNu_t, 1);
+}
+
+void orc_les_debug(orc_ctx *c, double *lm, double *mm) {
+  c->dbg_lm = lm;
+  c->dbg_mm = mm;
 }
 
 void orc_les_update(orc_ctx *c) {

@@ -60,6 +60,7 @@ def lib():
         L.orc_norms.argtypes = [C.c_void_p, dp]
         L.orc_les_gradients.argtypes = [C.c_void_p, dp, dp]
         L.orc_les_gradients_literal.argtypes = [C.c_void_p, dp, dp]
+        L.orc_les_debug.argtypes = [C.c_void_p, dp, dp]
         for name in ("orc_weno_ql", "orc_weno_qr", "orc_muscl_ql", "orc_muscl_qr"):
             f = getattr(L, name)
             f.restype = C.c_double
@@ -186,6 +187,15 @@ class Oracle:
         out = np.empty(2)
         self.L.orc_norms(self.h, _ptr(out))
         return out
+
+    def les_update_debug(self):
+        """les_update() and return (LM, MM) of the dynamic model."""
+        lm = np.zeros(self.shape)
+        mm = np.zeros(self.shape)
+        self.L.orc_les_debug(self.h, _ptr(lm), _ptr(mm))
+        self.L.orc_les_update(self.h)
+        self.L.orc_les_debug(self.h, None, None)
+        return lm, mm
 
     def les_gradients(self, literal=False):
         A = np.empty(self.shape + (9,))
