@@ -118,6 +118,9 @@ int check_params(const cles_params *p) {
   for (int d = 0; d < 3; d++)
     if ((p->bctype[2 * d] == 7) != (p->bctype[2 * d + 1] == 7))
       return set_error(nullptr, CLES_EINVAL, "periodic direction %d needs bctype 7 on both sides", d);
+  for (int q = 0; q < 6; q++)
+    if (!supported_bc(q, p->bctype[q]))
+      return set_error(nullptr, CLES_EINVAL, "bctype[%d]=%d is not on this path", q, p->bctype[q]);
   if (p->fluxorder < 1 || p->fluxorder > 3) return set_error(nullptr, CLES_EINVAL, "fluxorder must be 1, 2 or 3");
   if (p->eno_order != 2 && p->eno_order != 3) return set_error(nullptr, CLES_EINVAL, "ENOorder must be 2 or 3");
   if (p->calc_sigma) return set_error(nullptr, CLES_EINVAL, "-calc_sigma (shock sensor) is not on this path");
@@ -217,6 +220,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
   TRY(dalloc(c, &c->zet, 3 * np));
   TRY(dalloc(c, &c->aj, np));
   TRY(dalloc(c, &c->nvert, np));
+  TRY(dalloc(c, &c->fil2, np));
   for (int b = 0; b < 3; b++) {
     TRY(dalloc(c, &c->Q[b], 5 * np));
     TRY(dalloc(c, &c->P[b], np));
@@ -228,8 +232,9 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
     TRY(dalloc(c, &c->cs, np));
     TRY(dalloc(c, &c->cs_alt, np));
     TRY(dalloc(c, &c->nut, np));
+    TRY(dalloc(c, &c->ua, 3 * np));
     if (p->les != 1 && p->les != 10) {
-      TRY(dalloc(c, &c->grad, 10 * np));
+      TRY(dalloc(c, &c->grad, 7 * np));
       TRY(dalloc(c, &c->lm, np));
       TRY(dalloc(c, &c->mm, np));
     }
@@ -251,7 +256,8 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
 void cles_destroy(cles_ctx *c) {
   if (!c) return;
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
-                    c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm};
+                    c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
+                    c->fil2, c->ua};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->dnorm) (void)hipFree(c->dnorm);

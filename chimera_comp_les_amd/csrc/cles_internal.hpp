@@ -149,6 +149,7 @@ struct cles_ctx {
   double *coor;             // 3 planes
   double *csi, *eta, *zet;  // 3 planes each
   double *aj, *nvert;       // 1 plane each
+  double *fil2;             // (pow(1/aj, 1/3))^2: squared LES filter width per node
   int has_ib;               // any nvert != 0
   // state: three conserved-variable buffers with their pressures
   double *Q[3], *P[3];
@@ -160,7 +161,8 @@ struct cles_ctx {
   size_t xfer_elems;
   // LES
   double *cs, *cs_alt, *nut;
-  double *grad;    // 10 planes: du_dx..dw_dz, |S|
+  double *grad;    // 7 planes: strain-rate components S11 S12 S13 S22 S23 S33, |S|
+  double *ua;      // 3 planes: velocity held by lUcat after the gradient loop
   double *lm, *mm;
   cles::SigmaSpec sig;
   // reductions
@@ -178,6 +180,8 @@ int launch_metrics(cles_ctx *c);
 int launch_eos_all(cles_ctx *c, int buf);
 int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const double coef[3]);
 int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const double coef[3]);
+// bctype b is implemented on side q (0..5 = i-, i+, j-, j+, k-, k+)
+bool supported_bc(int side, int b);
 int launch_les(cles_ctx *c);
 int set_error(cles_ctx *c, int code, const char *fmt, ...);
 }  // namespace cles

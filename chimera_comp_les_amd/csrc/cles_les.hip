@@ -29,9 +29,11 @@ struct LesArgs {
   const double *qs;  // state of the last CalcUCat, or null (lUcat still zero)
   const double *csi, *eta, *zet, *aj, *nv;
   int has_ib;
-  double *grad;  // 10 planes: du_dx du_dy du_dz dv_dx dv_dy dv_dz dw_dx dw_dy dw_dz |S|
+  double *grad;  // 7 planes: S11 S12 S13 S22 S23 S33 (no trace removal) and |S|
+  double *ua;    // 3 planes: lUcat after the gradient loop
   double *lm, *mm;
   double *cs, *cs_out, *nut;
+  const double *fil2;  // per-node (pow(1/aj, 1/3))^2, formed once at setup
   int les, testfilter_ik, ucat_updated;
   double max_cs;
 };
@@ -54,13 +56,6 @@ __device__ __forceinline__ void u_stale(const LesArgs &a, long long n, double u[
   if (a.qs) vel(a.qs, n, a.g.np, u);
   else u[0] = u[1] = u[2] = 0.0;
 }
-// lUcat after the gradient loop has finished
-__device__ __forceinline__ void u_after(const LesArgs &a, int k, int j, int i, double u[3]) {
-  const long long n = nid(a.g, k, j, i);
-  if (a.ucat_updated && in_loop(a, k, j, i)) vel(a.q, n, a.g.np, u);
-  else u_stale(a, n, u);
-}
-
 // Compute_du_center + Compute_du_dxyz (les.c:31-63)
 __device__ __forceinline__ void grad9(const LesArgs &a, long long n, const double up_i[3], const double um_i[3],
                                       const double up_j[3], const double um_j[3], const double up_k[3],
@@ -92,15 +87,27 @@ __device__ __forceinline__ double sabs_of(const double g[9]) {
                      Szz * Szz));
 }
 
-__global__ void les_grad_kernel(LesArgs a) {
+// Velocity gradients of the loop les.c:250-309 (mixed current / previous
+// velocities, see above), stored as the six strain-rate components the test
+// filter uses (S_ab = (du_a/dx_b + du_b/dx_a)/2, les.c:520-522) and |S| with
+// the trace term removed (les.c:268-274).  Also stores the velocity lUcat
+// holds once the loop has finished, read by the test filter and by nu_t.
+__global__ __launch_bounds__(256) void les_grad_kernel(LesArgs a) {
   const Grid &g = a.g;
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (long long)g.mx * g.my * g.mz) return;
   const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
   const long long np = g.np;
-  if (!in_loop(a, k, j, i)) {
+  const bool loop = a.ucat_updated && in_loop(a, k, j, i);
+  double u[3];
+  if (loop) vel(a.q, n, np, u); else u_stale(a, n, u);
+  a.ua[n] = u[0];
+  a.ua[n + np] = u[1];
+  a.ua[n + 2 * np] = u[2];
+  if (!a.ucat_updated) return;
+  if (!loop) {
 #pragma unroll
-    for (int c = 0; c < 10; c++) a.grad[n + c * np] = 0.0;
+    for (int c = 0; c < 7; c++) a.grad[n + c * np] = 0.0;
     return;
   }
   double up_i[3], up_j[3], up_k[3], um_i[3], um_j[3], um_k[3];
@@ -112,9 +119,13 @@ __global__ void les_grad_kernel(LesArgs a) {
   if (in_loop(a, k - 1, j, i)) vel(a.q, n - g.sz, np, um_k); else u_stale(a, n - g.sz, um_k);
   double gr[9];
   grad9(a, n, up_i, um_i, up_j, um_j, up_k, um_k, gr);
-#pragma unroll
-  for (int c = 0; c < 9; c++) a.grad[n + c * np] = gr[c];
-  a.grad[n + 9 * np] = sabs_of(gr);
+  a.grad[n] = 0.5 * (gr[0] + gr[0]);
+  a.grad[n + np] = 0.5 * (gr[1] + gr[3]);
+  a.grad[n + 2 * np] = 0.5 * (gr[2] + gr[6]);
+  a.grad[n + 3 * np] = 0.5 * (gr[4] + gr[4]);
+  a.grad[n + 4 * np] = 0.5 * (gr[5] + gr[7]);
+  a.grad[n + 5 * np] = 0.5 * (gr[8] + gr[8]);
+  a.grad[n + 6 * np] = sabs_of(gr);
 }
 
 // gradient component as seen by the test filter: the local vectors lSx..lSz
@@ -128,7 +139,7 @@ __device__ __forceinline__ long long seam_image(const Grid &g, int k, int j, int
   return -1;
 }
 
-__global__ void les_filter_kernel(LesArgs a) {
+__global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a) {
   const Grid &g = a.g;
   const long long np = g.np;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -142,55 +153,50 @@ __global__ void les_filter_kernel(LesArgs a) {
     return;
   }
   // running sums of the 27-point weighted means (integrate_testfilter,
-  // les.c:80-131): summation order [k][j][i]
+  // les.c:80-131) in the reference's [k][j][i] summation order; S_ab is
+  // symmetric, so S21/S31/S32 reuse the S12/S13/S23 sums (identical operands)
   double s_rho = 0, s_ru = 0, s_rv = 0, s_rw = 0, s_ruu = 0, s_ruv = 0, s_ruw = 0, s_rvv = 0, s_rvw = 0, s_rww = 0;
-  double sS[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sSS[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double sS[6] = {0, 0, 0, 0, 0, 0}, sSS[6] = {0, 0, 0, 0, 0, 0};
   double wsum = 0;
-  double Sx[9];
+  double Sx[6];
+#pragma unroll 1
   for (int R = 0; R < 3; R++)
+#pragma unroll 1
     for (int Qd = 0; Qd < 3; Qd++)
+#pragma unroll
       for (int Pd = 0; Pd < 3; Pd++) {
         const int kk = k + R - 1, jj = j + Qd - 1, ii = i + Pd - 1;
         const long long n = nid(g, kk, jj, ii);
         const double w = 1. / (fabs(R - 1.0) + 1) * 1. / (fabs(Qd - 1.0) + 1) * 1. / (fabs(Pd - 1.0) + 1);
-        double u[3];
-        u_after(a, kk, jj, ii, u);
+        const double u0 = a.ua[n], u1 = a.ua[n + np], u2 = a.ua[n + 2 * np];
         const double rho = a.q[n], ru = a.q[n + np], rv = a.q[n + 2 * np], rw = a.q[n + 3 * np];
-        double G[9], S;
+        double Sv[6], S;
         if (interior(g, kk, jj, ii)) {
 #pragma unroll
-          for (int c = 0; c < 9; c++) G[c] = a.grad[n + c * np];
-          S = a.grad[n + 9 * np];
+          for (int c = 0; c < 6; c++) Sv[c] = a.grad[n + c * np];
+          S = a.grad[n + 6 * np];
         } else {
           const long long s = seam_image(g, kk, jj, ii);
-          if (s >= 0) {
 #pragma unroll
-            for (int c = 0; c < 9; c++) G[c] = a.grad[s + c * np];
-          } else {
-#pragma unroll
-            for (int c = 0; c < 9; c++) G[c] = 0.0;
-          }
+          for (int c = 0; c < 6; c++) Sv[c] = s >= 0 ? a.grad[s + c * np] : 0.0;
           S = 0.0;  // |S| has no periodic copy (lS, les.c:225-330)
         }
-        const double Sxx = 0.5 * (G[0] + G[0]), Sxy = 0.5 * (G[1] + G[3]), Sxz = 0.5 * (G[2] + G[6]);
-        const double Syy = 0.5 * (G[4] + G[4]), Syz = 0.5 * (G[5] + G[7]), Szz = 0.5 * (G[8] + G[8]);
-        const double Sv[9] = {Sxx, Sxy, Sxz, Sxy, Syy, Syz, Sxz, Syz, Szz};
         if (R == 1 && Qd == 1 && Pd == 1) {
 #pragma unroll
-          for (int c = 0; c < 9; c++) Sx[c] = Sv[c];
+          for (int c = 0; c < 6; c++) Sx[c] = Sv[c];
         }
         s_rho += w * rho;
         s_ru += w * ru;
         s_rv += w * rv;
         s_rw += w * rw;
-        s_ruu += w * (ru * u[0]);
-        s_ruv += w * (ru * u[1]);
-        s_ruw += w * (ru * u[2]);
-        s_rvv += w * (rv * u[1]);
-        s_rvw += w * (rv * u[2]);
-        s_rww += w * (rw * u[2]);
+        s_ruu += w * (ru * u0);
+        s_ruv += w * (ru * u1);
+        s_ruw += w * (ru * u2);
+        s_rvv += w * (rv * u1);
+        s_rvw += w * (rv * u2);
+        s_rww += w * (rw * u2);
 #pragma unroll
-        for (int c = 0; c < 9; c++) {
+        for (int c = 0; c < 6; c++) {
           sS[c] += w * Sv[c];
           sSS[c] += w * (Sv[c] * S * rho);
         }
@@ -199,6 +205,8 @@ __global__ void les_filter_kernel(LesArgs a) {
   const double filter = pow(1. / a.aj[n0], 1. / 3.);
   const double test_filter = a.testfilter_ik ? pow(5.0, 1. / 3.) * filter : 2.0 * filter;
   const double _rho = s_rho / wsum, _rhou = s_ru / wsum, _rhov = s_rv / wsum, _rhow = s_rw / wsum;
+  // full 3x3 tensors in row-major order from the six symmetric components
+  const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
   double Lij[9];
   Lij[0] = s_ruu / wsum - _rhou * _rhou / _rho;
   Lij[1] = s_ruv / wsum - _rhou * _rhov / _rho;
@@ -209,15 +217,15 @@ __global__ void les_filter_kernel(LesArgs a) {
   Lij[6] = s_ruw / wsum - _rhow * _rhou / _rho;
   Lij[7] = s_rvw / wsum - _rhow * _rhov / _rho;
   Lij[8] = s_rww / wsum - _rhow * _rhow / _rho;
-  double Sh[9], SSh[9];
+  double Sh6[6], SSh6[6];
 #pragma unroll
-  for (int c = 0; c < 9; c++) {
-    Sh[c] = sS[c] / wsum;
-    SSh[c] = sSS[c] / wsum;
+  for (int c = 0; c < 6; c++) {
+    Sh6[c] = sS[c] / wsum;
+    SSh6[c] = sSS[c] / wsum;
   }
   double S_hat = 0;
 #pragma unroll
-  for (int c = 0; c < 9; c++) S_hat += pow(Sh[c], 2.);
+  for (int c = 0; c < 9; c++) S_hat += pow(Sh6[sym[c]], 2.);
   S_hat = sqrt(2 * S_hat);
   double L_cat = 0.0, M_cat = 0.0;
   const double tf2 = pow(test_filter, 2.), f2 = pow(filter, 2.);
@@ -225,17 +233,17 @@ __global__ void les_filter_kernel(LesArgs a) {
   for (int p = 0; p < 3; p++)
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-      const int ab = p * 3 + q, aa = p * 4, bb = q * 4;
-      M_cat += -2.0 * _rho * tf2 * S_hat * (Sh[ab] * Sx[ab] - 0.33 * Sh[aa] * Sx[bb]) +
-               2.0 * f2 * (SSh[ab] * Sx[ab] - 0.33 * SSh[aa] * Sx[bb]);
-      L_cat += Lij[ab] * Sx[ab] - 0.33 * Sx[aa] * Lij[bb];
+      const int ab = sym[p * 3 + q], aa = sym[p * 4], bb = sym[q * 4];
+      M_cat += -2.0 * _rho * tf2 * S_hat * (Sh6[ab] * Sx[ab] - 0.33 * Sh6[aa] * Sx[bb]) +
+               2.0 * f2 * (SSh6[ab] * Sx[ab] - 0.33 * SSh6[aa] * Sx[bb]);
+      L_cat += Lij[p * 3 + q] * Sx[ab] - 0.33 * Sx[aa] * Lij[q * 4];
     }
   a.lm[n0] = L_cat;
   a.mm[n0] = M_cat;
 }
 
 // <LM>/<MM> and clipping on every owned node (les.c:889-930, 1146-1161)
-__global__ void les_cs_kernel(LesArgs a) {
+__global__ __launch_bounds__(256) void les_cs_kernel(LesArgs a) {
   const Grid &g = a.g;
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (long long)g.mx * g.my * g.mz) return;
@@ -281,7 +289,7 @@ __device__ __forceinline__ double lcs_at(const LesArgs &a, int k, int j, int i) 
 }
 
 // volume-weighted smoothing of Cs over 27 nodes (les.c:1258-1337)
-__global__ void les_smooth_kernel(LesArgs a) {
+__global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs a) {
   const Grid &g = a.g;
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (long long)g.mx * g.my * g.mz) return;
@@ -302,7 +310,7 @@ __global__ void les_smooth_kernel(LesArgs a) {
 }
 
 // nu_t = Cs Delta^2 |S| (les.c:1536-1573)
-__global__ void les_nut_kernel(LesArgs a) {
+__global__ __launch_bounds__(256) void les_nut_kernel(LesArgs a) {
   const Grid &g = a.g;
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (long long)g.mx * g.my * g.mz) return;
@@ -311,21 +319,26 @@ __global__ void les_nut_kernel(LesArgs a) {
     a.nut[n] = 0.0;
     return;
   }
+  const long long np = g.np;
+  auto ld = [&](long long m, double u[3]) {
+    u[0] = a.ua[m];
+    u[1] = a.ua[m + np];
+    u[2] = a.ua[m + 2 * np];
+  };
   double up_i[3], up_j[3], up_k[3], um_i[3], um_j[3], um_k[3];
-  u_after(a, k, j, i + 1, up_i);
-  u_after(a, k, j + 1, i, up_j);
-  u_after(a, k + 1, j, i, up_k);
-  u_after(a, k, j, i - 1, um_i);
-  u_after(a, k, j - 1, i, um_j);
-  u_after(a, k - 1, j, i, um_k);
+  ld(n + 1, up_i);
+  ld(n + g.sy, up_j);
+  ld(n + g.sz, up_k);
+  ld(n - 1, um_i);
+  ld(n - g.sy, um_j);
+  ld(n - g.sz, um_k);
   double gr[9];
   grad9(a, n, up_i, um_i, up_j, um_j, up_k, um_k, gr);
   const double Sabs = sabs_of(gr);
-  const double filter = pow(1. / a.aj[n], 1. / 3.);
-  a.nut[n] = a.cs[n] * pow(filter, 2.0) * Sabs;
+  a.nut[n] = a.cs[n] * a.fil2[n] * Sabs;
 }
 
-__global__ void fill_kernel(double *x, long long n, double v) {
+__global__ __launch_bounds__(256) void fill_kernel(double *x, long long n, double v) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) x[t] = v;
 }
@@ -340,7 +353,7 @@ int launch_les(cles_ctx *c) {
   a.qs = c->ucat_src >= 0 ? c->Q[c->ucat_src] : nullptr;
   a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert;
   a.has_ib = c->has_ib;
-  a.grad = c->grad; a.lm = c->lm; a.mm = c->mm;
+  a.grad = c->grad; a.lm = c->lm; a.mm = c->mm; a.ua = c->ua; a.fil2 = c->fil2;
   a.cs = c->cs; a.cs_out = c->cs_alt; a.nut = c->nut;
   a.les = p.les; a.testfilter_ik = p.testfilter_ik; a.max_cs = p.max_cs;
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
@@ -352,13 +365,16 @@ int launch_les(cles_ctx *c) {
   const bool early = (c->ti < 2 && p.tistart == 0 && !p.rstart) || p.les == 10;
   a.ucat_updated = 0;
   if (early || p.les == 1) {
-    // les.c:139-151: Cs = 0 (first step of a fresh run) or 0.03 everywhere
+    // les.c:139-151: Cs = 0 (first step of a fresh run) or 0.03 everywhere;
+    // lUcat keeps the velocity of the last CalcUCat
     hipLaunchKernelGGL(fill_kernel, dim3(gN), dim3(bs), 0, c->stream, c->cs, N, early ? 0.0 : 0.03);
     CLES_HIP(c, hipGetLastError());
-  } else {
     hipLaunchKernelGGL(les_grad_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
     CLES_HIP(c, hipGetLastError());
+  } else {
     a.ucat_updated = 1;
+    hipLaunchKernelGGL(les_grad_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(les_filter_kernel, dim3(gI), dim3(bs), 0, c->stream, a);
     CLES_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(les_cs_kernel, dim3(gN), dim3(bs), 0, c->stream, a);

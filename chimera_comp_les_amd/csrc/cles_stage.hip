@@ -39,7 +39,7 @@ struct StageArgs {
   const double *qold;   // Qold (5 planes)
   double *qo, *po;      // stage output
   double *rhs;          // parity mode output (5 planes)
-  const double *csi, *eta, *zet, *aj, *nv, *nut;
+  const double *csi, *eta, *zet, *aj, *nv, *nut, *fil2;
   int bc[6];
   int fluxorder, eno, central, iso, has_ib;
   double ren, Ma, Pr, gamma, Tref, ratio, dt;
@@ -360,7 +360,7 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
     if (Tr < 0)
       nu = T(1. / a.ren);
     else
-      nu = T(1. / a.ren * (1 + C_sutherland / a.Tref) / (Tr + C_sutherland / a.Tref) * pow(Tr, 1.5));
+      nu = T(1. / a.ren * (1 + C_sutherland / a.Tref) / (Tr + C_sutherland / a.Tref) * (Tr * sqrt(Tr)));
   }
   T div = T(2. / 3.) * (dudc * csi0 + dude * eta0 + dudz * zet0 + dvdc * csi1 + dvde * eta1 + dvdz * zet1 +
                         dwdc * csi2 + dwde * eta2 + dwdz * zet2);
@@ -402,8 +402,7 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
     T tz = (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * m0 + r23 * m1 + r33 * m2 - div * m2) * ajc * (nu_t * rho_c);
     T S2 = T(.5) * ((r11 - div) * (r11 - div) + r12 * r12 + r13 * r13 + r21 * r21 + (r22 - div) * (r22 - div) +
                     r23 * r23 + r31 * r31 + r32 * r32 + (r33 - div) * (r33 - div));
-    double filter = pow(1. / a.aj[gA], 1. / 3.);
-    T f2 = T(pow(filter, 2.0));
+    T f2 = T(a.fil2[gA]);  // (pow(1/aj, 1/3))^2 at the face's lower node
     T Ssx = -S2 * f2 * CI * T(2.0) / T(3.0) * m0 * rho_c * sigma_c;
     T Ssy = -S2 * f2 * CI * T(2.0) / T(3.0) * m1 * rho_c * sigma_c;
     T Ssz = -S2 * f2 * CI * T(2.0) / T(3.0) * m2 * rho_c * sigma_c;
@@ -625,6 +624,7 @@ int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const doubl
   a.po = out >= 0 ? c->P[out] : nullptr;
   a.rhs = c->rhs;
   a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert; a.nut = c->nut;
+  a.fil2 = c->fil2;
   for (int q = 0; q < 6; q++) a.bc[q] = p.bctype[q];
   a.fluxorder = p.fluxorder; a.eno = p.eno_order; a.central = p.central; a.iso = p.isothermal;
   a.has_ib = c->has_ib;

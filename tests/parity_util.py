@@ -54,3 +54,30 @@ def perturbed(Q, seed, amp=1e-3):
     rng = np.random.default_rng(seed)
     f = 1.0 + amp * rng.standard_normal(Q.shape[:3] + (1,))
     return cases.fill_seams(Q * f)
+
+
+def channel_case(nx=8, ny=10, nz=12, bc=(7, 7, 1, 1, 45, 8), iso=0, les=0, invicid=0, Ma=0.3, seed=0):
+    """Stretched channel: periodic in i, walls in j, inflow/outflow in k (the only
+    orientation the reference supports), uniform flow along k with a seeded
+    perturbation."""
+    rng = np.random.default_rng(seed)
+    xi = (np.arange(nx + 2) - 1.0) / nx
+    eta = np.linspace(0.0, 1.0, ny + 2)
+    y = 0.5 * (1 + np.tanh(2.2 * (eta - 0.5)) / np.tanh(1.1))   # tanh clustering at the walls
+    z = np.linspace(0.0, 2.0, nz + 2)
+    Z, Y, X = np.meshgrid(z, y, xi, indexing="ij")
+    coor = np.stack([X, Y, Z], axis=-1)
+    gamma = 1.4
+    rho = 1.0 + 1e-2 * rng.standard_normal(X.shape)
+    w = 1.0 + 1e-2 * rng.standard_normal(X.shape)
+    u = 1e-2 * rng.standard_normal(X.shape)
+    v = 1e-2 * rng.standard_normal(X.shape)
+    p = 1.0 / (gamma * Ma * Ma) * (1.0 + 1e-3 * rng.standard_normal(X.shape))
+    Q = cases.conserved(rho, u, v, w, p, gamma)
+    # node-periodic in i
+    Q[:, :, 0] = Q[:, :, -2]
+    Q[:, :, -1] = Q[:, :, 1]
+    cfg = dict(mx=nx + 2, my=ny + 2, mz=nz + 2, bctype=tuple(bc), invicid=invicid, les=les, fluxorder=3,
+               gamma=gamma, Ma=Ma, ren=300.0, Pr=0.72, sutherland=1, isothermal=iso, Twall=1.0,
+               dt=2e-4)
+    return cfg, coor, Q

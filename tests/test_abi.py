@@ -46,3 +46,29 @@ def test_default_params_match_reference_defaults():
     assert p.ren == 3000.0 and p.Ma == 2.81 and p.Pr == 0.72 and p.gamma == 1.4   # init.c:85-96
     assert p.dt == 0.01 and p.fluxorder == 3 and p.eno_order == 2 and p.central == 2
     assert p.max_cs == 0.05 and p.Tref == 273.15 and p.sigma_thrsh == 1e-5
+
+
+def _create_err(**kw):
+    L = cl.load_library()
+    base = dict(mx=8, my=9, mz=10, bctype=(7, 7, 7, 7, 7, 7))
+    base.update(kw)
+    p = cl.make_params(base)
+    err = ctypes.c_int(0)
+    coor = (ctypes.c_double * 3)()
+    ctx = L.cles_create(ctypes.byref(p), coor, ctypes.byref(err))
+    assert not ctx
+    return err.value, L.cles_last_error(None).decode()
+
+
+def test_create_rejects_unsupported_options_before_device_use():
+    # parameter validation runs on the host before any allocation
+    e, msg = _create_err(bctype=(7, 7, 1, 1, 47, 8))
+    assert e == -1 and "bctype[4]=47" in msg
+    e, msg = _create_err(bctype=(7, 3, 7, 7, 7, 7))
+    assert e == -1 and "periodic" in msg
+    e, msg = _create_err(fluxorder=4)
+    assert e == -1 and "fluxorder" in msg
+    e, msg = _create_err(les=2, invicid=1)
+    assert e == -1 and "LES" in msg
+    e, msg = _create_err(calc_sigma=1)
+    assert e == -1 and "calc_sigma" in msg

@@ -122,3 +122,26 @@ def test_norms_match():
     g.close()
     # the oracle's VecNorm(Q) is taken before the last stage's update, like the GPU's
     assert abs(ng[0] - no[0]) <= 1e-12 * no[0]
+
+
+@pytest.mark.parametrize("bc,iso,les", [
+    ((7, 7, 1, 1, 45, 8), 0, 0),
+    ((7, 7, 1, 1, 45, 8), 1, 0),
+    ((7, 7, 1, 1, 45, 8), 2, 0),
+    ((7, 7, 1, 1, 55, 44), 0, 2),
+    ((7, 7, 3, 3, 10, 49), 0, 0),
+    ((3, 3, 10, 10, 3, 45), 0, 0),
+    ((7, 7, 1, 1, 52, 45), 2, 2),
+])
+def test_steps_channel_boundaries(bc, iso, les):
+    from parity_util import channel_case
+    cfg, coor, Q = channel_case(bc=bc, iso=iso, les=les)
+    o, g = _pair(cfg, coor, Q)
+    o.step(3)
+    g.advance(3)
+    Qc, Pc = o.state()
+    Qg, Pg = g.state()
+    g.close()
+    assert np.all(np.isfinite(Qc))
+    assert state_err(Qg, Qc) <= STATE_TOL
+    assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL
