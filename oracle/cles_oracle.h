@@ -68,6 +68,18 @@ typedef struct {
 
 typedef struct orc_ctx orc_ctx;
 
+/* One interpolation node of the immersed body (IBMInfo, variables.h). */
+typedef struct {
+  int ni, nj, nk;            /* the node                                  */
+  int cell;                  /* nearest surface triangle                  */
+  int imode;                 /* ICP triangle, or -100 for a fallback      */
+  int i1, j1, k1, i2, j2, k2, i3, j3, k3; /* donor nodes                  */
+  double pmin[3];            /* nearest surface point                     */
+  double d_s, d_i;           /* node-surface and node-donor distances     */
+  double cs1, cs2, cs3;      /* barycentric weights of pmin               */
+  double cr1, cr2, cr3;      /* barycentric weights of the donor point    */
+} orc_ibm_info;
+
 orc_ctx *orc_create(const orc_params *p, const double *coor_global);
 void orc_destroy(orc_ctx *c);
 void orc_set_threads(int n);
@@ -89,8 +101,28 @@ void orc_form_rhs(orc_ctx *c, int istage, double *rhs); /* rhs.c:4703 */
 void orc_form_bcs(orc_ctx *c, double alfa);             /* rhs.c:477  */
 void orc_runge_kutta(orc_ctx *c);                       /* solvers.c:223 */
 void orc_les_update(orc_ctx *c);  /* les.c:135 + les.c:1477 */
-void orc_flow_step(orc_ctx *c);   /* main.c:900-974 one step, no IB */
+void orc_flow_step(orc_ctx *c);   /* main.c:900-974 one step */
 void orc_norms(const orc_ctx *c, double out[2]);       /* solvers.c:251-252 */
+
+/* Immersed body.  Classification + interpolation list
+ * (ibm_search_advanced, ibm_iso.c:87-795) for one triangulated surface;
+ * vertices x_bp,y_bp,z_bp, triangles nv1..nv3 (0-based); nvert_o may be NULL.
+ * Returns the number of interpolation nodes, or <0 on failure (-1 too many
+ * ray crossings, -2 no nearest triangle, -3 no donor stencil). After it,
+ * orc_runge_kutta applies the interpolation and FormBCS(0) after the stages
+ * (solvers.c:279-286). */
+int orc_ibm_search(orc_ctx *c, int n_v, const double *x_bp, const double *y_bp, const double *z_bp,
+                   int n_elmt, const int *nv1, const int *nv2, const int *nv3, const double *nvert_o);
+int orc_ibm_count(const orc_ctx *c);
+void orc_ibm_get_list(const orc_ctx *c, orc_ibm_info *out);
+/* surface velocities u, uold: 3 per vertex (zero after orc_ibm_search) */
+void orc_ibm_set_velocity(orc_ctx *c, const double *u, const double *uold);
+void orc_ibm_interpolation(orc_ctx *c, int add_dundt); /* ibm_iso.c:2572 */
+/* geometry kernels, exported for known-answer tests */
+int orc_intsect_triangle(const double orig[3], const double dir[3], const double v0[3],
+                         const double v1[3], const double v2[3], double *t, double *u, double *v);
+int orc_point_in_triangle(orc_v3 p, orc_v3 p1, orc_v3 p2, orc_v3 p3, double nfx, double nfy, double nfz);
+double orc_glibc_first_rand(int seed);  /* srand(seed); rand() of the C library */
 
 /* component building blocks exported for known-answer tests */
 double orc_weno_ql(double qm, double q0, double qp);

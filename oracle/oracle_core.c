@@ -232,6 +232,9 @@ void orc_destroy(orc_ctx *c) {
   free(c->CS); free(c->lCS); free(c->Nu_t); free(c->lNu_t); free(c->lUcat);
   free(c->fp1); free(c->fp2); free(c->fp3); free(c->conv); free(c->visc);
   free(c->al1); free(c->al2); free(c->al3);
+  free(c->x_bp); free(c->y_bp); free(c->z_bp); free(c->nf_x); free(c->nf_y); free(c->nf_z);
+  free(c->radvec); free(c->nv1); free(c->nv2); free(c->nv3); free(c->qvec); free(c->u_bp);
+  free(c->uold_bp); free(c->ibl);
   free(c);
 }
 

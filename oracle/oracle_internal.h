@@ -25,6 +25,14 @@ struct orc_ctx {
   orc_q *fp1, *fp2, *fp3, *conv, *visc;
   orc_v3 *al1, *al2, *al3;
   double *dbg_lm, *dbg_mm; /* caller-owned copies of LM/MM for tests */
+  /* immersed body (one surface, ibi = 0) */
+  int immersed;
+  int n_v, n_elmt;
+  double *x_bp, *y_bp, *z_bp, *nf_x, *nf_y, *nf_z, *radvec;
+  int *nv1, *nv2, *nv3;
+  orc_v3 *qvec, *u_bp, *uold_bp;
+  orc_ibm_info *ibl;     /* interpolation list in traversal order */
+  int n_ibl;
 };
 
 #define GI(c, k, j, i) ((((size_t)(k)) * (size_t)(c)->my + (size_t)(j)) * (size_t)(c)->mx + (size_t)(i))

@@ -677,7 +677,7 @@ void orc_form_rhs(orc_ctx *c, int istage, double *rhs_out) {
   if (rhs_out) memcpy(rhs_out, c->Rhs, c->nG * sizeof(orc_q));
 }
 
-/* RungeKutta, solvers.c:223-294 (no immersed body). */
+/* RungeKutta, solvers.c:223-294. */
 void orc_runge_kutta(orc_ctx *c) {
   const orc_params *P = &c->p;
   double alfa[3] = {1., 1 / 4., 2 / 3.}, beta[3] = {1., 3 / 4., 1 / 3.};
@@ -699,6 +699,10 @@ void orc_runge_kutta(orc_ctx *c) {
     orc_form_bcs(c, alfa[istage]);
     orc_g2l(c, (double *)c->Q, (double *)c->lQ, 5);
     orc_eq_of_state(c);
+  }
+  if (c->immersed) { /* solvers.c:279-286 */
+    orc_ibm_interpolation(c, 1);
+    orc_form_bcs(c, 0.);
   }
 }
 
