@@ -46,11 +46,26 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "fp32bulk"], default="fp64")
     ap.add_argument("--case", choices=["tgv", "vortex"], default="tgv")
     ap.add_argument("--cpu-N", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprofv3 FETCH/WRITE summary used for roofline.traffic")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 512 MiB device-to-device copy three times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
     return ap.parse_args()
+
+
+def copy_probe():
+    import torch
+    n = 64 << 20  # 64 Mi doubles = 512 MiB
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}), flush=True)
+    return 0
 
 
 def cpu_baseline(args, workload):
@@ -83,6 +98,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        return copy_probe()
     import chimera_comp_les_amd as cl
     from chimera_comp_les_amd import cases
     fluxorder = 3 if args.scheme == "weno" else 2

@@ -56,8 +56,22 @@ EXPORTS = [
     "cles_get_state", "cles_get_metrics", "cles_get_les", "cles_eq_of_state", "cles_form_rhs",
     "cles_runge_kutta", "cles_les_update", "cles_flow_step", "cles_advance", "cles_norms",
     "cles_get_time", "cles_set_time", "cles_time_stage_kernel", "cles_time_les",
-    "cles_device_bytes", "cles_last_error",
+    "cles_device_bytes", "cles_last_error", "cles_get_nvert", "cles_ibm_search",
+    "cles_ibm_set_velocity", "cles_ibm_get_list",
 ]
+
+IBM_INFO_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
+
+
+class CIbmInfo(C.Structure):
+    """cles_ibm_info of include/cles.h."""
+    _fields_ = [(n, C.c_int) for n in IBM_INFO_FIELDS] + [("pmin", C.c_double * 3)] + [
+        (n, C.c_double) for n in ("d_s", "d_i", "cs1", "cs2", "cs3", "cr1", "cr2", "cr3")]
+
+
+IBM_INFO_DTYPE = np.dtype(
+    [(n, np.int32) for n in IBM_INFO_FIELDS] + [("pmin", np.float64, (3,))]
+    + [(n, np.float64) for n in ("d_s", "d_i", "cs1", "cs2", "cs3", "cr1", "cr2", "cr3")], align=True)
 
 _LIB = None
 
@@ -98,6 +112,11 @@ def load_library(path=LIB_PATH):
     L.cles_device_bytes.restype = C.c_size_t
     L.cles_last_error.argtypes = [vp]
     L.cles_last_error.restype = C.c_char_p
+    ip = C.POINTER(C.c_int)
+    L.cles_get_nvert.argtypes = [vp, dp]
+    L.cles_ibm_search.argtypes = [vp, C.c_int, dp, dp, dp, C.c_int, ip, ip, ip, dp, ip]
+    L.cles_ibm_set_velocity.argtypes = [vp, dp, dp]
+    L.cles_ibm_get_list.argtypes = [vp, C.POINTER(CIbmInfo), C.c_int]
     _LIB = L
     return L
 
@@ -246,3 +265,37 @@ class Solver:
 
     def device_bytes(self):
         return int(self.L.cles_device_bytes(self.h))
+
+    # ------------------------------------------------------------ immersed body
+    def nvert(self):
+        out = np.empty(self.shape)
+        self._check(self.L.cles_get_nvert(self.h, _ptr(out)))
+        return out
+
+    def ibm_search(self, verts, tris, nvert_o=None):
+        """Classify the grid against a closed triangulated surface (verts
+        (n_v, 3), tris (n_elmt, 3) 0-based) and build the interpolation
+        stencils; returns the number of interpolation nodes."""
+        v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).T)
+        t = np.ascontiguousarray(np.asarray(tris, dtype=np.int32).T)
+        n = C.c_int(0)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        no = None if nvert_o is None else np.ascontiguousarray(nvert_o, dtype=np.float64)
+        self._check(self.L.cles_ibm_search(self.h, v.shape[1], _ptr(v[0]), _ptr(v[1]), _ptr(v[2]), t.shape[1],
+                                           ip(t[0]), ip(t[1]), ip(t[2]), _ptr(no), C.byref(n)))
+        self._n_ib = n.value
+        return n.value
+
+    def ibm_set_velocity(self, u, uold):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        uold = np.ascontiguousarray(uold, dtype=np.float64)
+        self._check(self.L.cles_ibm_set_velocity(self.h, _ptr(u), _ptr(uold)))
+
+    def ibm_list(self):
+        n = getattr(self, "_n_ib", 0)
+        out = np.zeros(n, dtype=IBM_INFO_DTYPE)
+        arr = (CIbmInfo * max(n, 1))()
+        self._check(self.L.cles_ibm_get_list(self.h, arr, n))
+        if n:
+            out[:] = np.frombuffer(arr, dtype=IBM_INFO_DTYPE, count=n)
+        return out

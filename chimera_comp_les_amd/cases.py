@@ -145,3 +145,98 @@ def tgv_case(N=256, steps=500, cfl=0.4, Re=1600.0, Ma=0.1, Pr=0.71, les=2, fluxo
 def interior_cells(cfg):
     """Unique interior nodes updated per stage (periodic seams not counted)."""
     return (cfg["mx"] - 2) * (cfg["my"] - 2) * (cfg["mz"] - 2)
+
+
+# ---------------------------------------------------------------- immersed body
+
+def xorshift64star(seed, n):
+    """n uniform numbers in [0, 1) from the xorshift64* generator
+    (x ^= x >> 12; x ^= x << 25; x ^= x >> 27; out = x * 2685821657736338717)."""
+    mask = (1 << 64) - 1
+    x = seed & mask or 1
+    out = np.empty(n)
+    for q in range(n):
+        x ^= x >> 12
+        x ^= (x << 25) & mask
+        x ^= x >> 27
+        r = (x * 2685821657736338717) & mask
+        out[q] = (r >> 11) * (1.0 / (1 << 53))
+    return out
+
+
+def icosphere(level, radius=0.5, center=(0.0, 0.0, 0.0), jitter=0.0, seed=20261203):
+    """Closed triangulated sphere: the icosahedron with every triangle split in
+    four `level` times (20 * 4**level triangles, 10 * 4**level + 2 vertices),
+    vertices on the sphere, triangles ordered so that (p2-p1) x (p3-p1) points
+    outward.  jitter > 0 moves every vertex coordinate by jitter * (u - 0.5),
+    u from xorshift64*(seed).  Returns (verts (n_v, 3), tris (n_elmt, 3))."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    tris = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+            (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+            (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        new = []
+        for a, b, c in tris:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            new += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        tris = new
+    V = np.array(verts) * radius
+    T = np.array(tris, dtype=np.int32)
+    cen = V[T].mean(axis=1)
+    nrm = np.cross(V[T[:, 1]] - V[T[:, 0]], V[T[:, 2]] - V[T[:, 0]])
+    flip = (nrm * cen).sum(axis=1) < 0
+    T[flip] = T[flip][:, [0, 2, 1]]
+    if jitter > 0:
+        V = V + jitter * (xorshift64star(seed, V.size).reshape(V.shape) - 0.5)
+    return V + np.asarray(center, dtype=np.float64), T
+
+
+def box_surface(lo, hi):
+    """Closed axis-aligned box as 12 triangles with outward normals."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    V = np.array([(x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0),
+                  (x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)], dtype=np.float64)
+    T = np.array([(0, 2, 1), (0, 3, 2), (4, 5, 6), (4, 6, 7), (0, 1, 5), (0, 5, 4),
+                  (2, 3, 7), (2, 7, 6), (1, 2, 6), (1, 6, 5), (3, 0, 4), (3, 4, 7)], dtype=np.int32)
+    return V, T
+
+
+def sphere_channel_case(nx=128, ny=128, nz=192, Lx=6.0, Ly=6.0, Lz=12.0, D=1.0, zc=3.0, level=5,
+                        jitter=1e-3, seed=20261203, Re=300.0, Ma=0.3, les=2, isothermal=2, cfl=0.4,
+                        bc=(7, 7, 1, 1, 45, 8)):
+    """Sphere of diameter D in a channel: x periodic along i (nx unique
+    nodes), walls at y = 0 and y = Ly along j (ny nodes, tanh clustering
+    with stretching 2.2), inflow at z = 0 and outflow at z = Lz along k (nz
+    nodes; the reference's inflow/outflow conditions act on k and rhoW).
+    Uniform inflow rho = 1, w = 1, T = 1 as the initial state; the sphere is
+    an icosphere of 20 * 4**level triangles jittered by jitter * D.
+    Returns (cfg, coor, Q, verts, tris)."""
+    x = (np.arange(nx + 2) - 1.0) * (Lx / nx)
+    eta = np.linspace(0.0, 1.0, ny)
+    y = Ly * 0.5 * (1 + np.tanh(2.2 * (eta - 0.5)) / np.tanh(1.1))
+    z = np.linspace(0.0, Lz, nz)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    coor = np.stack([X, Y, Z], axis=-1)
+    p0 = 1.0 / (GAMMA * Ma * Ma)
+    one = np.ones(X.shape)
+    Q = conserved(one, 0 * one, 0 * one, one, p0 * one, GAMMA)
+    verts, tris = icosphere(level, 0.5 * D, (0.5 * Lx, 0.5 * Ly, zc), jitter * D, seed)
+    h = min(Lx / nx, float(np.diff(y).min()), Lz / (nz - 1))
+    cfg = dict(mx=nx + 2, my=ny, mz=nz, bctype=tuple(bc), invicid=0, les=les, fluxorder=3, gamma=GAMMA,
+               Ma=Ma, ren=Re / D, Pr=0.72, sutherland=1, isothermal=isothermal, Twall=1.0,
+               dt=cfl * h / (1.0 + 1.0 / Ma))
+    return cfg, coor, Q, verts, tris

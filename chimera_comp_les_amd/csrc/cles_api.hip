@@ -257,9 +257,10 @@ void cles_destroy(cles_ctx *c) {
   if (!c) return;
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->ua};
+                    c->fil2, c->ua, c->qsnap};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
+  ibm_free(c);
   if (c->dnorm) (void)hipFree(c->dnorm);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   free(c);
@@ -280,6 +281,11 @@ int cles_set_nvert(cles_ctx *c, const double *nvert) {
   (void)hipFree(flag);
   c->has_ib = h;
   return CLES_OK;
+}
+
+int cles_get_nvert(cles_ctx *c, double *nvert) {
+  if (!c || !nvert) return CLES_EINVAL;
+  return download(c, c->nvert, nvert, 1, CLES_LAYOUT_GLOBAL);
 }
 
 int cles_set_state(cles_ctx *c, const double *Q, int layout) {
@@ -397,6 +403,14 @@ static int runge_kutta_async(cles_ctx *c) {
       c->ucat_les_mixed = 0;
     }
     in = out;
+  }
+  if (c->immersed) {  // solvers.c:279-286
+    int r = launch_ibm_interp(c, in, c->cur, 1);
+    if (r) return r;
+    r = launch_snapshot(c, in);
+    if (r) return r;
+    r = launch_boundary_post(c, in);
+    if (r) return r;
   }
   c->cur = in;  // Qold <- Q (main.c:971)
   return CLES_OK;

@@ -393,6 +393,25 @@ __global__ __launch_bounds__(256) void seam_eos_kernel(BndArgs a) {
   a.po[n] = (a.gamma - 1) * (qn[4] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) / qn[0]);
 }
 
+// periodic seams after the interpolation of the immersed body: Q and P both
+// come from the snapshot taken before FormBCS(0) (the reference reads the
+// ghost entries of lQ and lP, which FormBCS never writes), and no equation of
+// state follows (solvers.c:283-285)
+__global__ __launch_bounds__(256) void seam_post_kernel(BndArgs a) {
+  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const Grid &g = a.g;
+  const long long np = g.np;
+  if (b >= a.nb) return;
+  int k, j, i;
+  bnd_decode(g, b, k, j, i);
+  const long long s = seam_src(g, k, j, i);
+  if (s < 0) return;
+  const long long n = nid(g, k, j, i);
+#pragma unroll
+  for (int q = 0; q < 5; q++) a.qo[n + q * np] = a.q[s + q * np];
+  a.po[n] = a.p[s];
+}
+
 struct EosArgs {
   const double *q;
   double *p;
@@ -458,6 +477,49 @@ int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const
   }
   hipLaunchKernelGGL(seam_eos_kernel, grid(a.nb), dim3(bs), 0, c->stream, a);
   CLES_HIP(c, hipGetLastError());
+  return CLES_OK;
+}
+
+int launch_boundary_post(cles_ctx *c, int buf) {
+  const cles_params &p = c->p;
+  const long long np = c->g.np;
+  BndArgs a;
+  a.g = c->g;
+  a.q = c->qsnap;
+  a.p = c->qsnap + 5 * np;
+  a.qold = c->Q[c->cur];
+  a.qo = c->Q[buf];
+  a.po = c->P[buf];
+  a.aj = c->aj; a.eta = c->eta; a.zet = c->zet; a.nv = c->nvert; a.coor = c->coor;
+  for (int q = 0; q < 6; q++) a.bc[q] = p.bctype[q];
+  a.iso = p.isothermal; a.twoD = p.twoD; a.has_ib = c->has_ib;
+  a.gamma = p.gamma; a.Ma = p.Ma; a.Twall = p.Twall; a.dt = p.dt; a.alfa = 0.;
+  a.a_rhs = 0.; a.a_old = 0.; a.a_cur = 0.;
+  a.istage = 0;
+  a.periodic = c->g.px || c->g.py || c->g.pz;
+  a.norms = nullptr;
+  const long long mx = c->g.mx, my = c->g.my, mz = c->g.mz;
+  a.nb = 2 * mx * my + 2 * mx * (mz - 2) + 2 * (my - 2) * (mz - 2);
+  const int bs = 256;
+  auto grid = [&](long long n) { return dim3((unsigned)((n + bs - 1) / bs)); };
+  auto np7 = [&](int q) { return p.bctype[q] != 7 && p.bctype[q] != 0; };
+  if (np7(0) || np7(1)) {
+    hipLaunchKernelGGL(bc_i_kernel, grid(my * mz), dim3(bs), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+  }
+  if (np7(2) || np7(3)) {
+    hipLaunchKernelGGL(bc_j_kernel, grid(mx * mz), dim3(bs), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+  }
+  if (np7(4) || np7(5)) {
+    const int ls = p.bctype[2] == 1 ? 1 : 0, le = p.bctype[3] == 1 ? (int)my - 1 : (int)my;
+    hipLaunchKernelGGL(bc_k_kernel, grid(mx * (le - ls)), dim3(bs), 0, c->stream, a, ls, le);
+    CLES_HIP(c, hipGetLastError());
+  }
+  if (a.periodic) {
+    hipLaunchKernelGGL(seam_post_kernel, grid(a.nb), dim3(bs), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+  }
   return CLES_OK;
 }
 

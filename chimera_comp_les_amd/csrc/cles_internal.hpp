@@ -140,6 +140,24 @@ struct SigmaSpec {
 // ---------------------------------------------------------------------------
 }  // namespace cles
 
+// Device data of the immersed body (one surface).
+struct cles_ibm_dev {
+  int nv, ne;
+  double *vx, *vy, *vz;  // vertices
+  int *tv;               // 3 planes of ne: vertex ids of each triangle
+  double *nf;            // 3 planes of ne: unit normals
+  double *qv, *rad;      // bounding spheres
+  double *u, *uold;      // 3 planes of nv: surface velocity now / one step ago
+  int n_ib;              // interpolation nodes, in the reference's list order
+  int *node, *cell, *imode;
+  int *don;              // 9 planes: donor (i,j,k) x 3
+  long long *dn;         // 3 planes: donor node ids
+  double *info;          // 11 planes: pmin xyz, d_s, d_i, cs1..3, cr1..3
+  double *nrm;           // 3 planes: wall normal used by the interpolation
+  double *w;             // per-sweep work planes
+  double *carry;         // Pa carried from one sweep to the next
+};
+
 struct cles_ctx {
   cles_params p;
   cles::Grid g;
@@ -151,6 +169,9 @@ struct cles_ctx {
   double *aj, *nvert;       // 1 plane each
   double *fil2;             // (pow(1/aj, 1/3))^2: squared LES filter width per node
   int has_ib;               // any nvert != 0
+  int immersed;             // a surface was classified: RK steps end with the interpolation
+  cles_ibm_dev *ibm;
+  double *qsnap;            // 6 planes: Q and P before FormBCS(0) of an immersed step
   // state: three conserved-variable buffers with their pressures
   double *Q[3], *P[3];
   int cur;       // buffer holding Q^n (= Qold)
@@ -183,6 +204,11 @@ int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const
 // bctype b is implemented on side q (0..5 = i-, i+, j-, j+, k-, k+)
 bool supported_bc(int side, int b);
 int launch_les(cles_ctx *c);
+// immersed body
+int launch_ibm_interp(cles_ctx *c, int buf, int qold_buf, int add_dundt);
+int launch_snapshot(cles_ctx *c, int buf);
+int launch_boundary_post(cles_ctx *c, int buf);
+void ibm_free(cles_ctx *c);
 int set_error(cles_ctx *c, int code, const char *fmt, ...);
 }  // namespace cles
 

@@ -93,6 +93,42 @@ void cles_destroy(cles_ctx *c);
 /* Immersed-boundary node flags (Nvert, one per node, global layout):
  * 0 fluid, 1 IB node, 3 solid (ibm_iso.c:762-769).  Default: all fluid. */
 int cles_set_nvert(cles_ctx *c, const double *nvert);
+/* Download Nvert (one per node, global layout). */
+int cles_get_nvert(cles_ctx *c, double *nvert);
+
+/* One interpolation node of the immersed body (IBMInfo, variables.h:46-89),
+ * in the order the reference's list visits them. */
+typedef struct cles_ibm_info {
+  int ni, nj, nk;        /* the node                                       */
+  int cell;              /* nearest surface triangle                       */
+  int imode;             /* donor triangle of the 3x3 face (0..7), or -100 */
+  int i1, j1, k1, i2, j2, k2, i3, j3, k3; /* donor nodes                    */
+  double pmin[3];        /* nearest surface point                          */
+  double d_s, d_i;       /* node-surface, node-donor-point distances       */
+  double cs1, cs2, cs3;  /* barycentric weights of pmin in the triangle    */
+  double cr1, cr2, cr3;  /* weights of the donor point in the donor nodes  */
+} cles_ibm_info;
+
+/* Classify the nodes against one closed triangulated surface and build the
+ * interpolation stencils: ibm_search_advanced (ibm_iso.c:87-795) for body 0
+ * with the surface as ibm_read_ucd leaves it (ibm_io.c:1530-1760).
+ * x_bp,y_bp,z_bp: n_v vertices; nv1,nv2,nv3: n_elmt triangles (0-based,
+ * outward normal by the right-hand rule); nvert_o: the previous Nvert for the
+ * phase-change test, or NULL.  Nvert is reset (values below 5) and replaced.
+ * With -isothermal 1 or 2 the boundary nodes get interpolation stencils; from
+ * then on every RungeKutta ends with ibm_interpolation_advanced and
+ * FormBCS(0) (solvers.c:279-286).  *n_ib (may be NULL) receives the number
+ * of interpolation nodes.  Fails with CLES_EINVAL when a ray needs more than
+ * 1024 directions or crosses more than 48 triangles, or a boundary node has
+ * no stencil. */
+int cles_ibm_search(cles_ctx *c, int n_v, const double *x_bp, const double *y_bp, const double *z_bp,
+                    int n_elmt, const int *nv1, const int *nv2, const int *nv3, const double *nvert_o,
+                    int *n_ib);
+/* Surface velocity now and one step ago (ibm->u, ibm->uold), 3 per vertex.
+ * Zero after cles_ibm_search (a body at rest). */
+int cles_ibm_set_velocity(cles_ctx *c, const double *u, const double *uold);
+/* Copy the n interpolation nodes (n must equal the count) to out. */
+int cles_ibm_get_list(cles_ctx *c, cles_ibm_info *out, int n);
 
 /* Upload the conserved state; sets Q, Qold and the pressure by EqOfState.
  * Replaces the tail of SetInitialCondition (rhs.c:424-433). */
@@ -111,7 +147,8 @@ int cles_eq_of_state(cles_ctx *c);
  * (rhs.c:3327).  rhs (5 per node, global layout) may be NULL. */
 int cles_form_rhs(cles_ctx *c, int istage, double *rhs);
 /* RungeKutta, solvers.c:223: fluxorder stages of formRHS, stage update,
- * FormBCS, EqOfState (no immersed body). */
+ * FormBCS, EqOfState; with an immersed body then the interpolation and
+ * FormBCS(0). */
 int cles_runge_kutta(cles_ctx *c);
 /* Compute_Smagorinsky_Constant_1 (les.c:135) followed by
  * Compute_eddy_viscosity_LES (les.c:1477). */

@@ -86,6 +86,7 @@ void orc_set_threads(int n);
 
 /* nvert (global, one per node) -> Nvert/lNvert */
 void orc_set_nvert(orc_ctx *c, const double *nvert);
+void orc_get_nvert(const orc_ctx *c, double *nvert);
 /* Q (global interleaved) -> Q, lQ, Qold, lQold, then EqOfState
  * (tail of SetInitialCondition, rhs.c:424-433). */
 void orc_set_state(orc_ctx *c, const double *Q);
@@ -108,7 +109,8 @@ void orc_norms(const orc_ctx *c, double out[2]);       /* solvers.c:251-252 */
  * (ibm_search_advanced, ibm_iso.c:87-795) for one triangulated surface;
  * vertices x_bp,y_bp,z_bp, triangles nv1..nv3 (0-based); nvert_o may be NULL.
  * Returns the number of interpolation nodes, or <0 on failure (-1 too many
- * ray crossings, -2 no nearest triangle, -3 no donor stencil). After it,
+ * ray crossings, -2 no nearest triangle, -3 no donor stencil, -4 no regular
+ * ray in 1024 directions). After it,
  * orc_runge_kutta applies the interpolation and FormBCS(0) after the stages
  * (solvers.c:279-286). */
 int orc_ibm_search(orc_ctx *c, int n_v, const double *x_bp, const double *y_bp, const double *z_bp,

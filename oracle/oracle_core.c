@@ -243,6 +243,8 @@ void orc_set_nvert(orc_ctx *c, const double *nvert) {
   orc_g2l(c, c->nvert, c->lnvert, 1);
 }
 
+void orc_get_nvert(const orc_ctx *c, double *nvert) { memcpy(nvert, c->nvert, c->nG * sizeof(double)); }
+
 void orc_set_time(orc_ctx *c, int ti) { c->ti = ti; }
 int orc_get_time(const orc_ctx *c) { return c->ti; }
 

@@ -196,15 +196,19 @@ double orc_glibc_first_rand(int seed) {
 }
 
 /* point_cell_advanced, ibm_iso.c:1284-1396.  Returns 4 (inside) or 0,
- * or -1 when more than ORC_MAX_HITS crossings are found on one ray. */
+ * -1 when more than ORC_MAX_HITS crossings are found on one ray, -4 when
+ * ORC_MAX_SEEDS directions were all singular (the reference retries without
+ * bound; on a surface whose edge lies along the line the endpoints move on,
+ * it never stops). */
 #define ORC_MAX_HITS 1000
+#define ORC_MAX_SEEDS 1024
 static int point_cell(orc_ctx *c, orc_v3 p, int ip, int jp, int kp, int ncx, int ncy, int ncz, double dcx,
                       double dcy, double xbp_min, double ybp_min, double zbp_max, const bucket *cell,
                       unsigned char *searched, int *touched) {
   double dt[ORC_MAX_HITS];
   orc_v3 dnn[ORC_MAX_HITS];
   int searchtimes = 0;
-  for (;;) {
+  while (searchtimes < ORC_MAX_SEEDS) {
     searchtimes++;
     int nintp = 0, ntouched = 0, singular = 0;
     double dir[3];
@@ -242,6 +246,7 @@ static int point_cell(orc_ctx *c, orc_v3 p, int ip, int jp, int kp, int ncx, int
     for (int z = 0; z < ntouched; z++) searched[touched[z]] = 0;
     if (!singular) return (nintp % 2) ? 4 : 0;
   }
+  return -4;
 }
 
 /* --------------------------------------------------------------- stencils */
@@ -442,7 +447,7 @@ int orc_ibm_search(orc_ctx *c, int n_v, const double *x_bp, const double *y_bp, 
           const int kc = (int)floor((p.z - zbp_min) / dcz);
           const int r = point_cell(c, p, ic, jc, kc, ncx, ncy, ncz, dcx, dcy, xbp_min, ybp_min, zbp_max, cell,
                                    searched, touched);
-          if (r < 0) { status = -1; break; }
+          if (r < 0) { status = r == -4 ? -4 : -1; break; }
           nvert[g] = pmax(nvert[g], (double)r);
           if (nvert[g] < 0) nvert[g] = 0;
         }

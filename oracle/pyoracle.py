@@ -13,6 +13,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 
+class V3(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
 class OrcParams(C.Structure):
     _fields_ = [
         ("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int),
@@ -29,6 +33,14 @@ class OrcParams(C.Structure):
         ("twoD", C.c_int), ("testfilter_ik", C.c_int),
         ("tistart", C.c_int), ("rstart", C.c_int),
     ]
+
+
+IBM_INFO_DTYPE = np.dtype(
+    [(n, np.int32) for n in ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2",
+                             "i3", "j3", "k3")]
+    + [("pmin", np.float64, (3,))]
+    + [(n, np.float64) for n in ("d_s", "d_i", "cs1", "cs2", "cs3", "cr1", "cr2", "cr3")],
+    align=True)
 
 
 def lib():
@@ -76,8 +88,27 @@ def lib():
         L.orc_fp4.restype = C.c_double
         L.orc_fp4.argtypes = [dp]
         L.orc_rusanov.argtypes = [dp, C.c_double, dp, C.c_double, dp, dp]
+        ip = C.POINTER(C.c_int)
+        L.orc_get_nvert.argtypes = [C.c_void_p, dp]
+        L.orc_ibm_search.restype = C.c_int
+        L.orc_ibm_search.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_int, ip, ip, ip, dp]
+        L.orc_ibm_count.restype = C.c_int
+        L.orc_ibm_count.argtypes = [C.c_void_p]
+        L.orc_ibm_get_list.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_ibm_set_velocity.argtypes = [C.c_void_p, dp, dp]
+        L.orc_ibm_interpolation.argtypes = [C.c_void_p, C.c_int]
+        L.orc_intsect_triangle.restype = C.c_int
+        L.orc_intsect_triangle.argtypes = [dp] * 5 + [dp] * 3
+        L.orc_point_in_triangle.restype = C.c_int
+        L.orc_point_in_triangle.argtypes = [V3] * 4 + [C.c_double] * 3
+        L.orc_glibc_first_rand.restype = C.c_double
+        L.orc_glibc_first_rand.argtypes = [C.c_int]
         _LIB = L
     return _LIB
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
 def _ptr(a):
@@ -203,3 +234,51 @@ class Oracle:
         f = self.L.orc_les_gradients_literal if literal else self.L.orc_les_gradients
         f(self.h, _ptr(A), _ptr(S))
         return A, S
+
+    def nvert(self):
+        nv = np.empty(self.shape)
+        self.L.orc_get_nvert(self.h, _ptr(nv))
+        return nv
+
+    def ibm_search(self, verts, tris, nvert_o=None):
+        """Classify the grid against a closed triangulated surface and build the
+        interpolation list.  verts (n_v, 3), tris (n_elmt, 3) 0-based."""
+        v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).T)
+        t = np.ascontiguousarray(np.asarray(tris, dtype=np.int32).T)
+        self._mesh = (v, t)
+        no = None if nvert_o is None else _ptr(np.ascontiguousarray(nvert_o, dtype=np.float64))
+        n = self.L.orc_ibm_search(self.h, v.shape[1], _ptr(v[0]), _ptr(v[1]), _ptr(v[2]), t.shape[1],
+                                  _iptr(t[0]), _iptr(t[1]), _iptr(t[2]), no)
+        if n < 0:
+            raise RuntimeError("orc_ibm_search failed (%d)" % n)
+        return n
+
+    def ibm_list(self):
+        n = self.L.orc_ibm_count(self.h)
+        out = np.zeros(n, dtype=IBM_INFO_DTYPE)
+        if n:
+            self.L.orc_ibm_get_list(self.h, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def ibm_set_velocity(self, u, uold):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        uold = np.ascontiguousarray(uold, dtype=np.float64)
+        self.L.orc_ibm_set_velocity(self.h, _ptr(u), _ptr(uold))
+
+    def ibm_interpolation(self, add_dundt=1):
+        self.L.orc_ibm_interpolation(self.h, int(add_dundt))
+
+
+def intsect_triangle(orig, d, v0, v1, v2):
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (orig, d, v0, v1, v2)]
+    t, u, v = (np.zeros(1) for _ in range(3))
+    r = lib().orc_intsect_triangle(*[_ptr(x) for x in a], _ptr(t), _ptr(u), _ptr(v))
+    return r, float(t[0]), float(u[0]), float(v[0])
+
+
+def point_in_triangle(p, p1, p2, p3, n):
+    return lib().orc_point_in_triangle(V3(*p), V3(*p1), V3(*p2), V3(*p3), *[float(x) for x in n])
+
+
+def glibc_first_rand(seed):
+    return lib().orc_glibc_first_rand(int(seed))

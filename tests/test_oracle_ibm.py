@@ -1,0 +1,312 @@
+"""Pins of the oracle's immersed-boundary restatement (ibm_iso.c, compgeom.c,
+wallfunction.c), on the CPU.
+
+Each check is chosen so that a plausible slip fails it:
+  * srand/rand: the published first output of glibc's rand() after srand(1);
+  * ray/triangle and point-in-triangle: hand-worked known answers;
+  * classification: brute force on a convex surface (inside <=> behind every
+    face plane), then the marking and islanding rules applied in numpy;
+  * nearest surface point: exact point-to-triangle distance by brute force;
+  * donor stencil: the donor point lies on the ray from the surface point
+    through the node, at distance d_i past the node;
+  * interpolation: a flat wall with fields linear in the wall distance: the
+    no-slip rule reproduces the velocity exactly and the pressure takes the
+    donor-point value;
+  * the isothermal-2 pressure chain: closed-form recurrence over the list in
+    the order the reference's prepended list visits it.
+"""
+import ctypes
+import ctypes.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import pyoracle as po  # noqa: E402
+from chimera_comp_les_amd import cases  # noqa: E402
+
+
+def test_glibc_first_rand():
+    # glibc: srand(1); rand() == 1804289383 (the generator's documented first value)
+    assert po.glibc_first_rand(1) == 1804289383
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    for seed in (2, 3, 17, 1000):
+        libc.srand(seed)
+        expect = libc.rand()
+        assert po.glibc_first_rand(seed) == expect
+
+
+def test_intsect_triangle_known_answers():
+    v0, v1, v2 = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    r, t, u, v = po.intsect_triangle([0.2, 0.3, -2.0], [0, 0, 1], v0, v1, v2)
+    assert r == 1 and t == pytest.approx(2.0) and u == pytest.approx(0.2) and v == pytest.approx(0.3)
+    # behind the origin: hit with t < 0 (the caller discards it)
+    r, t, _, _ = po.intsect_triangle([0.2, 0.3, 1.0], [0, 0, 1], v0, v1, v2)
+    assert r == 1 and t == pytest.approx(-1.0)
+    assert po.intsect_triangle([0.8, 0.8, -1.0], [0, 0, 1], v0, v1, v2)[0] == 0   # outside, u+v > 1
+    assert po.intsect_triangle([-0.1, 0.2, -1.0], [0, 0, 1], v0, v1, v2)[0] == 0  # u < 0
+    assert po.intsect_triangle([0.2, 0.2, 1.0], [1, 0, 0], v0, v1, v2)[0] == 0    # parallel
+    s = 1 / np.sqrt(3)
+    r, t, _, _ = po.intsect_triangle([0, 0, 0], [s, s, s], [1, 0, 0], [0, 1, 0], [0, 0, 1])
+    assert r == 1 and t == pytest.approx(1 / np.sqrt(3))
+
+
+def test_point_in_triangle_projections():
+    tri = ((0, 0, 0), (1, 0, 0), (0, 1, 0))
+    assert po.point_in_triangle((0.2, 0.2, 5), *tri, (0, 0, 1)) == 1
+    assert po.point_in_triangle((0.7, 0.7, 0), *tri, (0, 0, 1)) == -1
+    assert po.point_in_triangle((0.5, 0.5, 0), *tri, (0, 0, 1)) == 1   # on the hypotenuse
+    tri_x = ((0, 0, 0), (0, 1, 0), (0, 0, 1))                            # plane x = 0, uses (z, y)
+    assert po.point_in_triangle((3, 0.2, 0.2), *tri_x, (1, 0, 0)) == 1
+    assert po.point_in_triangle((0, 0.9, 0.9), *tri_x, (1, 0, 0)) == -1
+    tri_y = ((0, 0, 0), (1, 0, 0), (0, 0, 1))                            # plane y = 0, uses (x, z)
+    assert po.point_in_triangle((0.1, 7, 0.1), *tri_y, (0, 1, 0)) == 1
+    assert po.point_in_triangle((-0.1, 0, 0.1), *tri_y, (0, 1, 0)) == -1
+
+
+# ------------------------------------------------------------------ helpers
+
+def small_sphere_case(level=2, jitter=0.0, iso=2):
+    return cases.sphere_channel_case(nx=16, ny=16, nz=24, Lx=2.4, Ly=2.4, Lz=3.2, zc=1.4, level=level,
+                                     jitter=jitter, isothermal=iso, les=0)
+
+
+def closest_point_on_triangles(p, A, B, C):
+    """Closest points of p on every triangle (A, B, C arrays (n, 3)); the
+    region test of the standard barycentric construction."""
+    ab, ac, ap = B - A, C - A, p - A
+    d1, d2 = (ab * ap).sum(1), (ac * ap).sum(1)
+    bp = p - B
+    d3, d4 = (ab * bp).sum(1), (ac * bp).sum(1)
+    cp = p - C
+    d5, d6 = (ab * cp).sum(1), (ac * cp).sum(1)
+    va = d3 * d6 - d5 * d4
+    vb = d5 * d2 - d1 * d6
+    vc = d1 * d4 - d3 * d2
+    out = np.empty_like(A)
+    done = np.zeros(len(A), bool)
+
+    def put(mask, val):
+        m = mask & ~done
+        out[m] = val[m]
+        done[:] |= m
+
+    put((d1 <= 0) & (d2 <= 0), A)
+    put((d3 >= 0) & (d4 <= d3), B)
+    v = np.where(np.abs(d1 - d3) > 0, d1 / np.where(d1 - d3 == 0, 1, d1 - d3), 0)[:, None]
+    put((vc <= 0) & (d1 >= 0) & (d3 <= 0), A + v * ab)
+    put((d6 >= 0) & (d5 <= d6), C)
+    w = np.where(d2 - d6 != 0, d2 / np.where(d2 - d6 == 0, 1, d2 - d6), 0)[:, None]
+    put((vb <= 0) & (d2 >= 0) & (d6 <= 0), A + w * ac)
+    den = (d4 - d3) + (d5 - d6)
+    w2 = np.where(den != 0, (d4 - d3) / np.where(den == 0, 1, den), 0)[:, None]
+    put((va <= 0) & ((d4 - d3) >= 0) & ((d5 - d6) >= 0), B + w2 * (C - B))
+    denom = 1.0 / np.where(va + vb + vc == 0, 1, va + vb + vc)
+    vv, ww = (vb * denom)[:, None], (vc * denom)[:, None]
+    put(np.ones(len(A), bool), A + ab * vv + ac * ww)
+    return out
+
+
+def brute_classification(coor, V, T, iso):
+    """nvert after ibm_search_advanced for a convex surface, by brute force."""
+    n = np.cross(V[T[:, 1]] - V[T[:, 0]], V[T[:, 2]] - V[T[:, 0]])
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    # convexity of the surface: every vertex behind every face plane
+    side = np.einsum("fvk,fk->fv", V[None, :, :] - V[T[:, 0]][:, None, :], n)
+    assert side.max() < 1e-12
+    X = coor.reshape(-1, 3)
+    s = (X[:, None, :] - V[T[:, 0]][None, :, :])
+    s = np.einsum("pfk,fk->pf", s, n)
+    assert np.abs(s).min() > 1e-9          # no node on a face plane
+    lab = np.where((s < 0).all(axis=1), 4, 0).reshape(coor.shape[:3])
+    mz, my, mx = lab.shape
+    pad = np.pad(lab, 1, mode="edge")      # clamped neighbours
+    nb4 = np.zeros(lab.shape, bool)
+    for dk in range(3):
+        for dj in range(3):
+            for di in range(3):
+                nb4 |= pad[dk:dk + mz, dj:dj + my, di:di + mx] == 4
+    marked = np.where((lab != 4) & nb4, 2, lab)
+    if iso in (1, 2):
+        lo = 0 if iso == 2 else 1
+        pm = np.pad(marked, 1, mode="edge")
+        around = np.zeros(lab.shape, int)
+        for off in ((0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0)):
+            sh = pm[1 + off[0]:1 + off[0] + mz, 1 + off[1]:1 + off[1] + my, 1 + off[2]:1 + off[2] + mx]
+            k_, j_, i_ = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+            # clamped neighbour equal to the node itself does not count (kk != k)
+            same = ((off[0] == 1) & (k_ == mz - 1)) | ((off[0] == -1) & (k_ == 0)) | \
+                   ((off[1] == 1) & (j_ == my - 1)) | ((off[1] == -1) & (j_ == 0)) | \
+                   ((off[2] == 1) & (i_ == mx - 1)) | ((off[2] == -1) & (i_ == 0))
+            around += ((sh > 1) & ~same).astype(int)
+        rng = np.zeros(lab.shape, bool)
+        rng[lo:mz - lo, lo:my - lo, lo:mx - lo] = True
+        marked = np.where(rng & (marked == 0) & (around > 3), 2, marked)
+    out = np.where(marked == 2, 1.0, np.where(marked == 4, 3.0, marked.astype(float)))
+    return out
+
+
+# -------------------------------------------------------------------- tests
+
+@pytest.mark.parametrize("iso", [2, 1])
+def test_classification_matches_brute_force(iso):
+    cfg, coor, Q, V, T = small_sphere_case(level=2, iso=iso)
+    o = po.Oracle(cfg, coor)
+    o.set_state(Q)
+    n = o.ibm_search(V, T)
+    nv = o.nvert()
+    ref = brute_classification(coor, V, T, iso)
+    assert np.array_equal(nv, ref)
+    assert n > 0 and (nv == 3).sum() > 0
+    L = o.ibm_list()
+    # every listed node is an IB node, and (iso 2) every IB node below k = mz-1 is listed
+    assert np.all(nv[L["nk"], L["nj"], L["ni"]] == 1.0)
+    if iso == 2:
+        assert len(L) == int((nv[:-1] == 1.0).sum())
+    # traversal order = reverse of the (k, j, i) scan order (AddIBMNode prepends)
+    lin = (L["nk"].astype(np.int64) * cfg["my"] + L["nj"]) * cfg["mx"] + L["ni"]
+    assert np.all(np.diff(lin) < 0)
+
+
+def test_nearest_point_and_donors_geometry():
+    cfg, coor, Q, V, T = small_sphere_case(level=2)
+    o = po.Oracle(cfg, coor)
+    o.set_state(Q)
+    o.ibm_search(V, T)
+    L = o.ibm_list()
+    A, B, C = V[T[:, 0]], V[T[:, 1]], V[T[:, 2]]
+    for b in L:
+        p = coor[b["nk"], b["nj"], b["ni"]]
+        cp = closest_point_on_triangles(p, A, B, C)
+        d = np.linalg.norm(cp - p, axis=1)
+        q = np.argmin(d)
+        assert b["d_s"] == pytest.approx(d[q], rel=1e-12, abs=1e-14)
+        assert np.allclose(b["pmin"], cp[q], rtol=0, atol=1e-12)
+        # barycentric weights of pmin in its triangle
+        t = T[b["cell"]]
+        rec = b["cs1"] * V[t[0]] + b["cs2"] * V[t[1]] + b["cs3"] * V[t[2]]
+        assert np.allclose(rec, b["pmin"], atol=1e-12)
+        # donor point on the ray from pmin through the node, d_i past the node
+        assert b["imode"] >= 0
+        X = [coor[b[kk], b[jj], b[ii]] for ii, jj, kk in (("i1", "j1", "k1"), ("i2", "j2", "k2"),
+                                                           ("i3", "j3", "k3"))]
+        qd = b["cr1"] * X[0] + b["cr2"] * X[1] + b["cr3"] * X[2]
+        nrm = (p - b["pmin"]) / np.linalg.norm(p - b["pmin"])
+        assert np.allclose(qd, p + b["d_i"] * nrm, atol=1e-11)
+        assert min(b["cr1"], b["cr2"], b["cr3"]) > -1e-9 and max(b["cr1"], b["cr2"], b["cr3"]) < 1 + 1e-9
+        # the three donors lie on one face of the node's 3x3x3 neighbourhood
+        D = np.array([[b[ii], b[jj], b[kk]] for ii, jj, kk in (("i1", "j1", "k1"), ("i2", "j2", "k2"),
+                                                               ("i3", "j3", "k3"))])
+        off = D - np.array([b["ni"], b["nj"], b["nk"]])
+        assert np.abs(off).max() == 1
+        assert any(np.all(off[:, a] == off[0, a]) and abs(off[0, a]) == 1 for a in range(3))
+
+
+def slab_case(iso):
+    """Periodic cartesian box; the body is a slab z < zw spanning x and y
+    beyond the grid.  Velocity and pressure are linear in z above it."""
+    N = 12
+    cfg = dict(mx=N + 2, my=N + 2, mz=N + 2, bctype=cases.PERIODIC, invicid=0, les=0, fluxorder=3,
+               gamma=1.4, Ma=0.3, ren=100.0, isothermal=iso, Twall=1.0, dt=1e-3)
+    coor, _ = cases.cartesian_grid(N, 1.0)
+    zw = 0.3 + 0.37 / N
+    # not square in x-y: the diagonal of each face must not run along the line
+    # on which the ray endpoints of a bucket column move (slope dcy/dcx)
+    V, T = cases.box_surface((-0.5, -0.3, -0.4), (1.5, 1.6, zw))
+    z = coor[..., 2]
+    u, v = 0.7 * (z - zw), -0.2 * (z - zw)
+    w = 0.0 * z
+    p = 1.0 / (1.4 * 0.09) * (1 + 0.1 * (z - zw))
+    rho = 1.0 + 0.05 * (z - zw)
+    Q = cases.conserved(rho, u, v, w, p, 1.4)
+    return cfg, coor, Q, V, T, zw
+
+
+def test_interpolation_exact_for_linear_fields_at_a_flat_wall():
+    cfg, coor, Q, V, T, zw = slab_case(iso=1)
+    o = po.Oracle(cfg, coor)
+    o.set_state(Q)
+    n = o.ibm_search(V, T)
+    L = o.ibm_list()
+    assert n > 0
+    top = L[np.isclose(L["pmin"][:, 2], zw) & (coor[L["nk"], L["nj"], L["ni"]][:, 2] > zw)]
+    assert len(top) > 0
+    Q0, P0 = o.state()
+    o.ibm_interpolation(add_dundt=1)
+    Q1, P1 = o.state()
+    for b in top:
+        k, j, i = b["nk"], b["nj"], b["ni"]
+        zn = coor[k, j, i, 2]
+        assert b["d_s"] == pytest.approx(zn - zw, abs=1e-13)
+        # p: the value interpolated at the donor point (zero normal gradient)
+        pz = 1.0 / (1.4 * 0.09) * (1 + 0.1 * (zn + b["d_i"] - zw))
+        assert P1[k, j, i] == pytest.approx(pz, rel=1e-13)
+        rho = P1[k, j, i] * 1.4 * 0.09          # isothermal 1: rho = p gamma Ma^2
+        assert Q1[k, j, i, 0] == pytest.approx(rho, rel=1e-13)
+        assert Q1[k, j, i, 1] == pytest.approx(rho * 0.7 * (zn - zw), rel=1e-12, abs=1e-15)
+        assert Q1[k, j, i, 2] == pytest.approx(rho * -0.2 * (zn - zw), rel=1e-12, abs=1e-15)
+
+
+def test_isothermal2_pressure_chain_follows_list_order():
+    """Fluid at rest with T != Twall; the donors are fluid nodes, so each
+    sweep's interpolated p is p0 and only the carried Pa couples the nodes:
+      sweep 0:  rho_n = rho0,  p_n = rho0 Tb_n / (g M^2)
+      sweep s:  Pa_n = p_{n-1} (Pa_0 = last p of sweep s-1),
+                rho_n = rho0 - dt L_n / a^2,  L_n = (-a / sc_n) (p0 - Pa_n),
+                p_n = rho_n Tb_n / (g M^2)
+    with Tb_n = (T0 - Tw) sb_n / sc_n + Tw, a = sqrt(Tw) / M."""
+    cfg, coor, Q, V, T, zw = slab_case(iso=2)
+    g, M, Tw, dt = 1.4, 0.3, 1.0, cfg["dt"]
+    rho0, T0 = 1.0, 1.3
+    p0 = rho0 * T0 / (g * M * M)
+    Q = cases.conserved(np.full(coor.shape[:3], rho0), 0 * coor[..., 0], 0 * coor[..., 0], 0 * coor[..., 0],
+                        np.full(coor.shape[:3], p0), g)
+    o = po.Oracle(cfg, coor)
+    o.set_state(Q)
+    o.ibm_search(V, T)
+    L = o.ibm_list()
+    nv = o.nvert()
+    don = np.stack([nv[L[k], L[j], L[i]] for i, j, k in (("i1", "j1", "k1"), ("i2", "j2", "k2"),
+                                                         ("i3", "j3", "k3"))])
+    fluid_donors = np.all(don < 0.5, axis=0)
+    assert fluid_donors.all()
+    o.ibm_interpolation(add_dundt=1)
+    Q1, P1 = o.state()
+    sb, sc = L["d_s"], L["d_s"] + L["d_i"]
+    Tb = (T0 - Tw) * sb / sc + Tw
+    a = np.sqrt(Tw / (M * M))
+    p = rho0 * Tb / (g * M * M)
+    rho = np.full(len(L), rho0)
+    for _ in range(4):
+        carry = p[-1]
+        newp = np.empty_like(p)
+        Pa = carry
+        for m in range(len(L)):
+            Lm = (0.0 - a) / sc[m] * (p0 - Pa)
+            rho[m] = rho0 - dt * Lm / a / a
+            newp[m] = rho[m] * Tb[m] / (g * M * M)
+            Pa = newp[m]
+        p = newp
+    got_p = P1[L["nk"], L["nj"], L["ni"]]
+    got_r = Q1[L["nk"], L["nj"], L["ni"], 0]
+    assert np.allclose(got_p, p, rtol=1e-13, atol=0)
+    assert np.allclose(got_r, rho, rtol=1e-13, atol=0)
+    # the chain matters: a per-node Pa (no carry) gives a different answer
+    assert np.abs(p - rho0 * Tb / (g * M * M)).max() > 1e-6 * p0
+
+
+def test_immersed_step_runs_and_stays_finite():
+    cfg, coor, Q, V, T = small_sphere_case(level=2, jitter=1e-3)
+    o = po.Oracle(cfg, coor)
+    o.set_state(Q)
+    o.ibm_search(V, T)
+    o.step(3)
+    Q1, P1 = o.state()
+    assert np.isfinite(Q1).all() and np.isfinite(P1).all()
+    nv = o.nvert()
+    # solid nodes keep their initial state (zero right-hand side)
+    assert np.array_equal(Q1[nv == 3], Q[nv == 3])
