@@ -5,6 +5,9 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <cmath>
+#include <vector>
+
 #include "cles_internal.hpp"
 
 namespace cles {
@@ -131,6 +134,26 @@ int check_params(const cles_params *p) {
   return CLES_OK;
 }
 
+// LES filter widths, formed once per grid on the host with the C library's
+// pow, the function the reference calls (rhs.c:1693, les.c:396-397, les.c:1517):
+// fil2 = pow(pow(1/aj, 1/3), 2) and the squared test-filter width.  The
+// squares are the exactly rounded products, as in the device kernels.
+int filter_widths(cles_ctx *c) {
+  const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
+  std::vector<double> aj((size_t)N), f2((size_t)N), t2((size_t)N);
+  int r = download(c, c->aj, aj.data(), 1, CLES_LAYOUT_GLOBAL);
+  if (r) return r;
+  const double c5 = pow(5.0, 1. / 3.);
+  for (long long n = 0; n < N; n++) {
+    const double filter = pow(1. / aj[(size_t)n], 1. / 3.);
+    const double test_filter = c->p.testfilter_ik ? c5 * filter : 2.0 * filter;
+    f2[(size_t)n] = filter * filter;
+    t2[(size_t)n] = test_filter * test_filter;
+  }
+  if ((r = upload(c, f2.data(), c->fil2, 1, CLES_LAYOUT_GLOBAL))) return r;
+  return upload(c, t2.data(), c->tfil2, 1, CLES_LAYOUT_GLOBAL);
+}
+
 void sigma_spec(cles_ctx *c) {
   const cles_params &p = c->p;
   c->sig.bnd[0] = p.s_x;
@@ -229,6 +252,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
   c->xfer_elems = (size_t)((g.mx + 4) * (long long)(g.my + 4) * (g.mz + 4)) * 5;
   TRY(dalloc(c, &c->xfer, c->xfer_elems));
   if (p->les) {
+    TRY(dalloc(c, &c->tfil2, np));
     TRY(dalloc(c, &c->cs, np));
     TRY(dalloc(c, &c->cs_alt, np));
     TRY(dalloc(c, &c->nut, np));
@@ -248,6 +272,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
   TRY(upload(c, coor, c->coor, 3, CLES_LAYOUT_GLOBAL));
   TRY(launch_metrics(c));
   TRY(hipStreamSynchronize(c->stream) == hipSuccess ? 0 : set_error(c, CLES_EDEVICE, "metrics"));
+  if (p->les) TRY(filter_widths(c));
 #undef TRY
   if (err) *err = CLES_OK;
   return c;
@@ -257,7 +282,7 @@ void cles_destroy(cles_ctx *c) {
   if (!c) return;
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->ua, c->qsnap};
+                    c->fil2, c->ua, c->qsnap, c->tfil2};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   ibm_free(c);

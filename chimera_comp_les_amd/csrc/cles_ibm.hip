@@ -13,9 +13,10 @@
 // operation order and the library is built without contraction, so they are
 // bit-identical to the reference's.  The interpolation carries its pressure
 // variable Pa from one list node to the next (the reference sets it only in
-// the first sweep, ibm_iso.c:2743); that serial chain is an affine
-// recurrence, evaluated here by one workgroup as an affine-map prefix scan
-// followed by a serial replay per chunk.
+// the first sweep, ibm_iso.c:2743); that serial chain is evaluated by one
+// workgroup: an affine-map prefix scan estimates the entry value of every
+// chunk of the list, exact serial replays of the chunks are then iterated to
+// the bit-identical serial result.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -685,8 +686,15 @@ __device__ __forceinline__ void write_node(const InterpArgs &a, int m, double rh
   a.p[n] = pb;
 }
 
-// list-ordered part of one sweep, one workgroup: the isothermal-2 Pa chain as
-// an affine prefix scan over contiguous chunks, then the writes
+// list-ordered part of one sweep, one workgroup.  In sweeps after the
+// first the isothermal-2 density of node m uses Pa = p of node m-1 (the
+// reference's function-scope Pa), a serial chain.  The list is cut into
+// one contiguous chunk per thread; the entry value of every chunk is first
+// estimated by an affine-map prefix scan, then each thread replays its chunk
+// with the exact serial arithmetic and the chunk entries are iterated until
+// none changes.  The serial sequence is the only fixed point of that
+// iteration and each pass makes at least one more chunk entry exact, so the
+// result is bit-identical to evaluating the list in order.
 __global__ __launch_bounds__(kChainThreads) void interp_chain_kernel(InterpArgs a) {
   __shared__ double sA[kChainThreads], sB[kChainThreads];
   const int t = threadIdx.x, nib = a.n_ib;
@@ -708,7 +716,7 @@ __global__ __launch_bounds__(kChainThreads) void interp_chain_kernel(InterpArgs 
     if (m1 == nib && m0 < m1) *a.carry = last;
     return;
   }
-  // affine map of each node: Pa_next = alpha + beta * Pa
+  // estimate: affine map of each node, Pa_next = alpha + beta * Pa
   const double a_sound = sqrt(a.Twall / (a.Ma * a.Ma));
   const double gm2 = a.gamma * a.Ma * a.Ma;
   double A = 0, B = 1;
@@ -735,8 +743,27 @@ __global__ __launch_bounds__(kChainThreads) void interp_chain_kernel(InterpArgs 
     __syncthreads();
   }
   const double carry = *a.carry;
-  double x = t == 0 ? carry : sA[t - 1] + sB[t - 1] * carry;
+  const double guess = t == 0 ? carry : sA[t - 1] + sB[t - 1] * carry;
   __syncthreads();
+  double *entry = sA;  // entry[t]: Pa entering chunk t
+  entry[t] = guess;
+  __syncthreads();
+  for (int pass = 0; pass <= kChainThreads; pass++) {
+    double x = entry[t];
+    for (int m = m0; m < m1; m++) {
+      double rhob, pb;
+      iso2_node(a, m, x, rhob, pb);
+      x = pb;
+    }
+    __syncthreads();
+    int changed = 0;
+    if (t + 1 < kChainThreads && __double_as_longlong(entry[t + 1]) != __double_as_longlong(x)) {
+      entry[t + 1] = x;
+      changed = 1;
+    }
+    if (!__syncthreads_or(changed)) break;
+  }
+  double x = entry[t];
   for (int m = m0; m < m1; m++) {
     double rhob, pb;
     iso2_node(a, m, x, rhob, pb);

@@ -106,6 +106,19 @@ template <typename T> __device__ __forceinline__ T fp4(const T u[4]) {
   return T(1.0 / 12.0) * (-u[0] + T(7) * u[1] + T(7) * u[2] - u[3]);
 }
 
+// pow(x, 1.5) of Sutherland's law (rhs.c:1636) rounded once: x*s plus the
+// exact low part of x*s and x*(sqrt(x) - s), s = RN(sqrt(x)); fma gives
+// x - s*s and the low half of x*s exactly, so the sum carries ~104 bits.
+__device__ __forceinline__ double pow15_cr(double x) {
+  if (x == 0.0) return 0.0;
+  const double s = sqrt(x);
+  const double r = fma(-s, s, x);
+  const double hi = x * s;
+  const double lo = fma(x, s, -hi);
+  const double corr = x * (r / (2.0 * s));
+  return hi + (lo + corr);
+}
+
 // Rusanov flux, gamma hard-coded to 1.4 as in rhs.c:2596 / 3256.
 template <typename T>
 __device__ __forceinline__ void rusanov(const T *L, const T *R, const T *h, T uconL, T uconR, T *F) {
@@ -168,6 +181,7 @@ struct cles_ctx {
   double *csi, *eta, *zet;  // 3 planes each
   double *aj, *nvert;       // 1 plane each
   double *fil2;             // (pow(1/aj, 1/3))^2: squared LES filter width per node
+  double *tfil2;            // squared test-filter width (2 or 5^(1/3) filter widths)
   int has_ib;               // any nvert != 0
   int immersed;             // a surface was classified: RK steps end with the interpolation
   cles_ibm_dev *ibm;

@@ -33,7 +33,8 @@ struct LesArgs {
   double *ua;    // 3 planes: lUcat after the gradient loop
   double *lm, *mm;
   double *cs, *cs_out, *nut;
-  const double *fil2;  // per-node (pow(1/aj, 1/3))^2, formed once at setup
+  const double *fil2;   // per-node (pow(1/aj, 1/3))^2, formed once at setup
+  const double *tfil2;  // per-node squared test-filter width
   int les, testfilter_ik, ucat_updated;
   double max_cs;
 };
@@ -202,8 +203,6 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a) {
         }
         wsum += w;
       }
-  const double filter = pow(1. / a.aj[n0], 1. / 3.);
-  const double test_filter = a.testfilter_ik ? pow(5.0, 1. / 3.) * filter : 2.0 * filter;
   const double _rho = s_rho / wsum, _rhou = s_ru / wsum, _rhov = s_rv / wsum, _rhow = s_rw / wsum;
   // full 3x3 tensors in row-major order from the six symmetric components
   const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
@@ -225,10 +224,10 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a) {
   }
   double S_hat = 0;
 #pragma unroll
-  for (int c = 0; c < 9; c++) S_hat += pow(Sh6[sym[c]], 2.);
+  for (int c = 0; c < 9; c++) S_hat += Sh6[sym[c]] * Sh6[sym[c]];
   S_hat = sqrt(2 * S_hat);
   double L_cat = 0.0, M_cat = 0.0;
-  const double tf2 = pow(test_filter, 2.), f2 = pow(filter, 2.);
+  const double tf2 = a.tfil2[n0], f2 = a.fil2[n0];
 #pragma unroll
   for (int p = 0; p < 3; p++)
 #pragma unroll
@@ -353,7 +352,7 @@ int launch_les(cles_ctx *c) {
   a.qs = c->ucat_src >= 0 ? c->Q[c->ucat_src] : nullptr;
   a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert;
   a.has_ib = c->has_ib;
-  a.grad = c->grad; a.lm = c->lm; a.mm = c->mm; a.ua = c->ua; a.fil2 = c->fil2;
+  a.grad = c->grad; a.lm = c->lm; a.mm = c->mm; a.ua = c->ua; a.fil2 = c->fil2; a.tfil2 = c->tfil2;
   a.cs = c->cs; a.cs_out = c->cs_alt; a.nut = c->nut;
   a.les = p.les; a.testfilter_ik = p.testfilter_ik; a.max_cs = p.max_cs;
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;

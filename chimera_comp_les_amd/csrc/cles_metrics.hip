@@ -109,12 +109,6 @@ __global__ void metric_copy_kernel(CopyArgs a) {
 
 // squared LES filter width (pow(1/aj, 1/3))^2 of rhs.c:1693 and les.c:1574,
 // formed once per node with the same two pow calls the reference makes
-__global__ void filter_width_kernel(const double *aj, double *fil2, long long n) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const double filter = pow(1. / aj[t], 1. / 3.);
-  fil2[t] = pow(filter, 2.0);
-}
 
 }  // namespace
 
@@ -131,8 +125,6 @@ int launch_metrics(cles_ctx *c) {
       hipLaunchKernelGGL(metric_copy_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, ca);
       CLES_HIP(c, hipGetLastError());
     }
-  hipLaunchKernelGGL(filter_width_kernel, dim3((unsigned)((N + bs - 1) / bs)), dim3(bs), 0, c->stream, c->aj, c->fil2, N);
-  CLES_HIP(c, hipGetLastError());
   return CLES_OK;
 }
 

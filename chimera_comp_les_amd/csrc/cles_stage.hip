@@ -360,7 +360,7 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
     if (Tr < 0)
       nu = T(1. / a.ren);
     else
-      nu = T(1. / a.ren * (1 + C_sutherland / a.Tref) / (Tr + C_sutherland / a.Tref) * (Tr * sqrt(Tr)));
+      nu = T(1. / a.ren * (1 + C_sutherland / a.Tref) / (Tr + C_sutherland / a.Tref) * pow15_cr(Tr));
   }
   T div = T(2. / 3.) * (dudc * csi0 + dude * eta0 + dudz * zet0 + dvdc * csi1 + dvde * eta1 + dvdz * zet1 +
                         dwdc * csi2 + dwde * eta2 + dwdz * zet2);

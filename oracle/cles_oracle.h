@@ -125,6 +125,7 @@ int orc_intsect_triangle(const double orig[3], const double dir[3], const double
                          const double v1[3], const double v2[3], double *t, double *u, double *v);
 int orc_point_in_triangle(orc_v3 p, orc_v3 p1, orc_v3 p2, orc_v3 p3, double nfx, double nfy, double nfz);
 double orc_glibc_first_rand(int seed);  /* srand(seed); rand() of the C library */
+double orc_pow15(double x);             /* x^1.5 rounded once (Sutherland law)  */
 
 /* component building blocks exported for known-answer tests */
 double orc_weno_ql(double qm, double q0, double qp);

@@ -676,3 +676,6 @@ void orc_ibm_interpolation(orc_ctx *c, int add_dundt) {
     orc_g2l(c, c->P, c->lP, 1);
   }
 }
+
+/* exported for the known-answer test of the exactly rounded x^1.5 */
+double orc_pow15(double x) { return orc_pow15_cr(x); }

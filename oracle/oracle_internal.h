@@ -44,6 +44,23 @@ struct orc_ctx {
 void orc_g2l(const orc_ctx *c, const double *g, double *l, int nc);
 void orc_l2l(const orc_ctx *c, double *l, int nc);
 
+/* pow(x, 1.5) and pow(x, 2.) of the reference, evaluated as their exact
+ * values rounded once (the C library's pow misrounds x^1.5 and x^2 in about
+ * 0.1% of arguments; the exactly rounded value is the one both this
+ * restatement and the device path use).  x^1.5 = x*s + x*(sqrt(x) - s) with
+ * s = RN(sqrt(x)): fma gives x - s*s and the low half of x*s exactly, so the
+ * sum carries ~104 bits before its single rounding. */
+static inline double orc_pow15_cr(double x) {
+  if (x == 0.0) return 0.0;
+  const double s = sqrt(x);
+  const double r = fma(-s, s, x);
+  const double hi = x * s;
+  const double lo = fma(x, s, -hi);
+  const double corr = x * (r / (2.0 * s));
+  return hi + (lo + corr);
+}
+static inline double orc_sq(double x) { return x * x; }
+
 /* PetscMin / PetscMax as macros in the reference (petscmath.h) */
 static inline double pmin(double a, double b) { return a < b ? a : b; }
 static inline double pmax(double a, double b) { return a < b ? b : a; }

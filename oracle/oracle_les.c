@@ -257,13 +257,13 @@ static void smagorinsky(orc_ctx *c) {
         }
         double S_hat = 0;
         for (int a = 0; a < 3; a++)
-          for (int b = 0; b < 3; b++) S_hat += pow(Sh[a][b], 2.);
+          for (int b = 0; b < 3; b++) S_hat += orc_sq(Sh[a][b]);
         S_hat = sqrt(2 * S_hat);
         double L_cat = 0.0, M_cat = 0.0;
         for (int a = 0; a < 3; a++)
           for (int b = 0; b < 3; b++) {
-            M_cat += -2.0 * _rho * pow(test_filter, 2.) * S_hat * (Sh[a][b] * Sxyz[a][b] - 0.33 * Sh[a][a] * Sxyz[b][b]) +
-                     2.0 * pow(filter, 2.) * (SSh[a][b] * Sxyz[a][b] - 0.33 * SSh[a][a] * Sxyz[b][b]);
+            M_cat += -2.0 * _rho * orc_sq(test_filter) * S_hat * (Sh[a][b] * Sxyz[a][b] - 0.33 * Sh[a][a] * Sxyz[b][b]) +
+                     2.0 * orc_sq(filter) * (SSh[a][b] * Sxyz[a][b] - 0.33 * SSh[a][a] * Sxyz[b][b]);
             L_cat += Lij[a][b] * Sxyz[a][b] - 0.33 * Sxyz[a][a] * Lij[b][b];
           }
         LM[n0] = L_cat;
@@ -358,7 +358,7 @@ static void eddy_viscosity(orc_ctx *c) {
         grad_at(c, NULL, &u[n + sx], &u[n + sy], &u[n + sz], &u[n - sx], &u[n - sy], &u[n - sz], n, gr);
         double Sabs = sabs_of(gr);
         double filter = pow(1. / c->laj[n], 1. / 3.);
-        c->Nu_t[g] = c->CS[g] * pow(filter, 2.0) * Sabs;
+        c->Nu_t[g] = c->CS[g] * orc_sq(filter) * Sabs;
       }
   orc_g2l(c, c->Nu_t, c->lNu_t, 1);
 }

@@ -534,7 +534,7 @@ static void face_visc(const orc_ctx *c, int d, int k, int j, int i, orc_q *F, or
     if (Tr < 0)
       nu = 1. / P->ren;
     else
-      nu = 1. / P->ren * (1 + C_sutherland / Tref) / (Tr + C_sutherland / Tref) * pow(Tr, 1.5);
+      nu = 1. / P->ren * (1 + C_sutherland / Tref) / (Tr + C_sutherland / Tref) * orc_pow15_cr(Tr);
   }
   double div = 2. / 3. * (dudc * csi0 + dude * eta0 + dudz * zet0 + dvdc * csi1 + dvde * eta1 + dvdz * zet1 +
                           dwdc * csi2 + dwde * eta2 + dwdz * zet2);
@@ -573,9 +573,9 @@ static void face_visc(const orc_ctx *c, int d, int k, int j, int i, orc_q *F, or
     double S2 = .5 * ((r11 - div) * (r11 - div) + r12 * r12 + r13 * r13 + r21 * r21 + (r22 - div) * (r22 - div) +
                       r23 * r23 + r31 * r31 + r32 * r32 + (r33 - div) * (r33 - div));
     double filter = pow(1. / c->laj[A], 1. / 3.);
-    double Ssx = -S2 * pow(filter, 2.0) * CI * 2.0 / 3.0 * m0 * rho_c * sigma_c;
-    double Ssy = -S2 * pow(filter, 2.0) * CI * 2.0 / 3.0 * m1 * rho_c * sigma_c;
-    double Ssz = -S2 * pow(filter, 2.0) * CI * 2.0 / 3.0 * m2 * rho_c * sigma_c;
+    double Ssx = -S2 * orc_sq(filter) * CI * 2.0 / 3.0 * m0 * rho_c * sigma_c;
+    double Ssy = -S2 * orc_sq(filter) * CI * 2.0 / 3.0 * m1 * rho_c * sigma_c;
+    double Ssz = -S2 * orc_sq(filter) * CI * 2.0 / 3.0 * m2 * rho_c * sigma_c;
     fu += tx + Ssx;
     fv += ty + Ssy;
     fw += tz + Ssz;

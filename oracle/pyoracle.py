@@ -101,6 +101,8 @@ def lib():
         L.orc_intsect_triangle.argtypes = [dp] * 5 + [dp] * 3
         L.orc_point_in_triangle.restype = C.c_int
         L.orc_point_in_triangle.argtypes = [V3] * 4 + [C.c_double] * 3
+        L.orc_pow15.restype = C.c_double
+        L.orc_pow15.argtypes = [C.c_double]
         L.orc_glibc_first_rand.restype = C.c_double
         L.orc_glibc_first_rand.argtypes = [C.c_int]
         _LIB = L
@@ -282,3 +284,7 @@ def point_in_triangle(p, p1, p2, p3, n):
 
 def glibc_first_rand(seed):
     return lib().orc_glibc_first_rand(int(seed))
+
+
+def pow15(x):
+    return lib().orc_pow15(float(x))

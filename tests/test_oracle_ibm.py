@@ -310,3 +310,22 @@ def test_immersed_step_runs_and_stays_finite():
     nv = o.nvert()
     # solid nodes keep their initial state (zero right-hand side)
     assert np.array_equal(Q1[nv == 3], Q[nv == 3])
+
+
+def test_pow15_is_exactly_rounded():
+    """Sutherland's pow(Tr, 1.5) is evaluated as x^1.5 rounded once; checked
+    against 60-digit decimal arithmetic (float() of a Decimal rounds to
+    nearest), and within one ulp of the C library's pow."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.pow.restype = ctypes.c_double
+    libm.pow.argtypes = [ctypes.c_double, ctypes.c_double]
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([rng.uniform(0.2, 4.0, 4000), [1.0, 4.0, 0.25, 2.0, 1e-300, 1e300 ** 0.5]])
+    for x in xs:
+        x = float(x)
+        exact = float(Decimal(x) * Decimal(x).sqrt())
+        assert po.pow15(x) == exact, x
+        assert abs(po.pow15(x) - libm.pow(x, 1.5)) <= np.spacing(exact)
+    assert po.pow15(0.0) == 0.0
