@@ -44,9 +44,10 @@ def parse():
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--scheme", choices=["weno", "muscl"], default="weno")
     ap.add_argument("--precision", choices=["fp64", "fp32bulk"], default="fp64")
-    ap.add_argument("--case", choices=["tgv", "vortex"], default="tgv")
+    ap.add_argument("--case", choices=["tgv", "vortex", "sphere"], default="tgv")
     ap.add_argument("--cpu-N", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="oracle steps of the CPU sample (default: 30, 8 for the sphere case)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprofv3 FETCH/WRITE summary used for roofline.traffic")
@@ -74,13 +75,22 @@ def cpu_baseline(args, workload):
     from chimera_comp_les_amd import cases
     threads = min(16, os.cpu_count() or 1)
     N = args.cpu_N
+    mesh = None
     if workload == "tgv":
         cfg, coor, Q = cases.tgv_case(N=N, fluxorder=3 if args.scheme == "weno" else 2)
+    elif workload == "sphere":
+        cfg, coor, Q, V, T = cases.sphere_channel_case()
+        mesh = (V, T)
+        N = "192x128x128"
     else:
         cfg, coor, Q = cases.vortex_case(N=N)
     o = pyoracle.Oracle(cfg, coor, threads=threads)
     o.set_state(Q)
+    if mesh is not None:
+        o.ibm_search(*mesh)
     o.step(1)  # warm-up (first step of a fresh run has Cs = 0)
+    if args.cpu_steps is None:
+        args.cpu_steps = 8 if workload == "sphere" else 30
     t0 = time.perf_counter()
     o.step(args.cpu_steps)
     dt = time.perf_counter() - t0
@@ -88,8 +98,9 @@ def cpu_baseline(args, workload):
     rate = args.cpu_steps * cfg["fluxorder"] * cells / dt
     o.close()
     return {"value": rate, "unit": "cell-stage/s", "cores": threads, "kind": "port",
-            "sample": "CPU restatement (oracle/, C, -O3, OpenMP %d threads), same workload at %d^3, "
-                      "%d steps after 1 warm-up step, %.2f s" % (threads, N, args.cpu_steps, dt)}
+            "sample": "CPU restatement (oracle/, C, -O3, OpenMP %d threads), same workload at %s, "
+                      "%d steps after 1 warm-up step, %.2f s" % (threads, N if isinstance(N, str) else "%d^3" % N,
+                                                               args.cpu_steps, dt)}
 
 
 def main():
@@ -108,6 +119,12 @@ def main():
         workload = ("taylor-green %d^3 Re1600 Ma0.1 Pr0.71 Sutherland dynamic-Smagorinsky "
                     "%s SSP-RK%d" % (args.N, "WENO3" if fluxorder == 3 else "MUSCL", fluxorder))
         b_stage = B_STAGE_LES
+    elif args.case == "sphere":
+        cfg, coor, Q, V, T = cases.sphere_channel_case()
+        workload = ("immersed sphere D=1 in a channel, 128x128x192 nodes (x periodic, y walls tanh 2.2, z "
+                    "inflow/outflow), icosphere 20480 triangles jittered 1e-3 D, Re_D 300 Ma 0.3 Sutherland "
+                    "dynamic-Smagorinsky isothermal-2 walls SSP-RK3")
+        b_stage = B_STAGE_LES
     else:
         cfg, coor, Q = cases.vortex_case(N=args.N)
         workload = "isentropic-vortex %d^3 curvilinear inviscid WENO3/Rusanov SSP-RK3" % args.N
@@ -115,6 +132,14 @@ def main():
     prec = cl.PREC_FP64 if args.precision == "fp64" else cl.PREC_FP32BULK
     s = cl.Solver(cfg, coor, precision=prec)
     s.set_state(Q)
+    extra = {}
+    if args.case == "sphere":
+        t_s = time.perf_counter()
+        n_ib = s.ibm_search(V, T)
+        nv = s.nvert()
+        extra = {"ib_nodes": n_ib, "solid_nodes": int((nv > 2.5).sum()),
+                 "fluid_cells": int((nv[1:-1, 1:-1, 1:-1] < 0.5).sum()),
+                 "ibm_search_s": time.perf_counter() - t_s}
     del Q, coor
     stages = cfg["fluxorder"]
     cells = cases.interior_cells(cfg)
@@ -136,7 +161,7 @@ def main():
         try:
             with open(args.traffic) as f:
                 tr = json.load(f)
-            key = "%s_%d_%s" % (args.case, args.N, args.precision)
+            key = "%s_%s_%s" % (args.case, args.N if args.case != "sphere" else "c4", args.precision)
             if key in tr:
                 traffic = tr[key]["bytes_per_launch"]
         except (OSError, ValueError, KeyError):
@@ -155,8 +180,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32-bulk/f64",
         "data": "synthetic",
-        "config": {"workload": workload, "nodes": [cfg["mx"], cfg["my"], cfg["mz"]], "cells": cells,
-                   "stages_per_step": stages, "dt": cfg["dt"], "precision": args.precision},
+        "config": dict({"workload": workload, "nodes": [cfg["mx"], cfg["my"], cfg["mz"]], "cells": cells,
+                        "stages_per_step": stages, "dt": cfg["dt"], "precision": args.precision}, **extra),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "stage_kernel", "kernel_ms": ms_stage, "bytes_per_cell": b_stage},

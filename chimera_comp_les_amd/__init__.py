@@ -27,6 +27,8 @@ LAYOUT_GLOBAL = 0
 LAYOUT_LOCAL = 1
 PREC_FP64 = 0
 PREC_FP32BULK = 1
+# return codes of include/cles.h
+OK, EINVAL, ENOMEM, EDEVICE, ESTATE = 0, -1, -2, -3, -4
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "out of device memory", -3: "HIP runtime error",
           -4: "call out of sequence"}
@@ -57,7 +59,10 @@ EXPORTS = [
     "cles_runge_kutta", "cles_les_update", "cles_flow_step", "cles_advance", "cles_norms",
     "cles_get_time", "cles_set_time", "cles_time_stage_kernel", "cles_time_les",
     "cles_device_bytes", "cles_last_error", "cles_get_nvert", "cles_ibm_search",
-    "cles_ibm_set_velocity", "cles_ibm_get_list",
+    "cles_ibm_set_velocity", "cles_ibm_get_list", "cles_integrals", "cles_footprint",
+    "cles_batch_create", "cles_batch_destroy", "cles_batch_add", "cles_batch_remove", "cles_batch_size",
+    "cles_batch_used", "cles_batch_set_tile", "cles_batch_advance", "cles_batch_time_step",
+    "cles_batch_last_error", "cles_set_tile", "cles_batch_launch", "cles_batch_wait",
 ]
 
 IBM_INFO_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
@@ -114,6 +119,26 @@ def load_library(path=LIB_PATH):
     L.cles_last_error.restype = C.c_char_p
     ip = C.POINTER(C.c_int)
     L.cles_get_nvert.argtypes = [vp, dp]
+    L.cles_integrals.argtypes = [vp, dp]
+    L.cles_footprint.argtypes = [C.POINTER(CParams)]
+    L.cles_footprint.restype = C.c_size_t
+    L.cles_batch_create.argtypes = [C.c_size_t, C.POINTER(C.c_int)]
+    L.cles_batch_create.restype = vp
+    L.cles_batch_destroy.argtypes = [vp]
+    L.cles_batch_destroy.restype = None
+    L.cles_batch_add.argtypes = [vp, vp]
+    L.cles_batch_remove.argtypes = [vp, vp]
+    L.cles_batch_size.argtypes = [vp]
+    L.cles_batch_used.argtypes = [vp]
+    L.cles_batch_used.restype = C.c_size_t
+    L.cles_batch_set_tile.argtypes = [vp, C.c_int]
+    L.cles_batch_advance.argtypes = [vp, C.c_int]
+    L.cles_batch_launch.argtypes = [vp, C.c_int]
+    L.cles_batch_wait.argtypes = [vp]
+    L.cles_batch_time_step.argtypes = [vp, C.c_int, dp]
+    L.cles_batch_last_error.argtypes = [vp]
+    L.cles_batch_last_error.restype = C.c_char_p
+    L.cles_set_tile.argtypes = [vp, C.c_int]
     L.cles_ibm_search.argtypes = [vp, C.c_int, dp, dp, dp, C.c_int, ip, ip, ip, dp, ip]
     L.cles_ibm_set_velocity.argtypes = [vp, dp, dp]
     L.cles_ibm_get_list.argtypes = [vp, C.POINTER(CIbmInfo), C.c_int]
@@ -266,6 +291,13 @@ class Solver:
     def device_bytes(self):
         return int(self.L.cles_device_bytes(self.h))
 
+    def integrals(self):
+        """{mass, momentum x/y/z, energy, kinetic energy} summed as Q/aj over
+        the unique interior nodes."""
+        out = np.empty(6)
+        self._check(self.L.cles_integrals(self.h, _ptr(out)))
+        return out
+
     # ------------------------------------------------------------ immersed body
     def nvert(self):
         out = np.empty(self.shape)
@@ -299,3 +331,77 @@ class Solver:
         if n:
             out[:] = np.frombuffer(arr, dtype=IBM_INFO_DTYPE, count=n)
         return out
+
+    def set_tile(self, tile):
+        """Stage-kernel tile shape: 0 32x4x2, 1 16x4x4, 2 16x8x2, 3 64x2x2."""
+        self._check(self.L.cles_set_tile(self.h, int(tile)))
+
+
+def footprint(cfg, precision=None):
+    """Device bytes a Solver for cfg allocates (the batch admission size)."""
+    p = make_params(cfg)
+    if precision is not None:
+        p.precision = precision
+    return load_library().cles_footprint(C.byref(p))
+
+
+class Batch:
+    """Independent periodic solves advanced together by the batched kernels.
+    Members are Solver objects with their state uploaded."""
+
+    def __init__(self, hbm_budget):
+        self.L = load_library()
+        err = C.c_int(0)
+        self.h = self.L.cles_batch_create(C.c_size_t(int(hbm_budget)), C.byref(err))
+        if not self.h:
+            raise CLESError("cles_batch_create failed (%d): %s" % (err.value, self.L.cles_last_error(None).decode()))
+        self.members = []
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CLESError("%s (%d)" % (self.L.cles_batch_last_error(self.h).decode(), rc))
+
+    def add(self, solver):
+        """Admit a member; returns False when it does not fit the HBM budget."""
+        rc = self.L.cles_batch_add(self.h, solver.h)
+        if rc == ENOMEM:
+            return False
+        self._check(rc)
+        self.members.append(solver)
+        return True
+
+    def remove(self, solver):
+        self._check(self.L.cles_batch_remove(self.h, solver.h))
+        self.members.remove(solver)
+
+    def used(self):
+        return self.L.cles_batch_used(self.h)
+
+    def set_tile(self, tile):
+        self._check(self.L.cles_batch_set_tile(self.h, int(tile)))
+
+    def advance(self, nsteps):
+        self._check(self.L.cles_batch_advance(self.h, int(nsteps)))
+
+    def launch(self, nsteps):
+        """queue nsteps batched steps and return (see wait)"""
+        self._check(self.L.cles_batch_launch(self.h, int(nsteps)))
+
+    def wait(self):
+        self._check(self.L.cles_batch_wait(self.h))
+
+    def time_step(self, nrep=5):
+        ms = C.c_double(0)
+        self._check(self.L.cles_batch_time_step(self.h, int(nrep), C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            self.L.cles_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -184,6 +184,24 @@ int reset_norms(cles_ctx *c) {
 }
 
 }  // namespace
+
+int rk_coef(const cles_params &p, int istage, double coef[3]) {
+  double alfa;
+  return rk_coefficients(p, istage, coef, &alfa);
+}
+
+// device bytes cles_create allocates for p (the allocation list of cles_create)
+size_t ctx_footprint(const cles_params *p) {
+  const long long N = (long long)p->mx * p->my * p->mz;
+  const size_t np = (size_t)((N + 63) / 64 * 64);
+  size_t planes = 3 + 9 + 1 + 1 + 1 + 15 + 3 + 5;  // coor, metrics, aj, nvert, fil2, Q x3, P x3, rhs
+  if (p->les) {
+    planes += 1 + 3 + 3;  // tfil2, cs, cs_alt, nut, ua
+    if (p->les != 1 && p->les != 10) planes += 7 + 2;  // strain rates, LM, MM
+  }
+  const size_t xfer = (size_t)((p->mx + 4) * (long long)(p->my + 4) * (p->mz + 4)) * 5;
+  return (planes * np + xfer) * sizeof(double);
+}
 }  // namespace cles
 
 using namespace cles;
@@ -323,6 +341,13 @@ int cles_set_state(cles_ctx *c, const double *Q, int layout) {
   r = launch_eos_all(c, b);
   if (r) return r;
   CLES_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->p.precision == CLES_PREC_FP32BULK) {
+    // reference state of the split fluxes: node (1,1,1) of this state
+    const long long n = nid(c->g, 1, 1, 1);
+    CLES_HIP(c, hipMemcpy(&c->ref_rho, c->Q[b] + n, sizeof(double), hipMemcpyDeviceToHost));
+    CLES_HIP(c, hipMemcpy(&c->ref_p, c->P[b] + n, sizeof(double), hipMemcpyDeviceToHost));
+    if (!(c->ref_rho > 0.0)) return set_error(c, CLES_EINVAL, "non-positive density at node (1,1,1)");
+  }
   c->have_state = 1;
   return CLES_OK;
 }
@@ -563,6 +588,12 @@ int cles_time_les(cles_ctx *c, int nrep, double *ms) {
 }
 
 size_t cles_device_bytes(const cles_ctx *c) { return c ? c->bytes : 0; }
+
+int cles_set_tile(cles_ctx *c, int tile) {
+  if (!c || tile < 0 || tile > 3) return CLES_EINVAL;
+  c->tile = tile;
+  return CLES_OK;
+}
 
 const char *cles_last_error(const cles_ctx *c) { return c ? c->err : g_err; }
 

@@ -10,6 +10,8 @@
 // the stage input when FormBCS runs.  The i, j and k passes of the reference
 // overwrite each other on edges and corners in that order; each boundary node
 // here takes the value of the last pass that covers it.
+#include <cstring>
+
 #include "cles_internal.hpp"
 
 namespace cles {
@@ -374,8 +376,7 @@ __global__ __launch_bounds__(256) void bc_k_kernel(BndArgs a, int ls, int le) {
 
 // periodic seams from the stage input and the equation of state on every
 // boundary node
-__global__ __launch_bounds__(256) void seam_eos_kernel(BndArgs a) {
-  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void seam_eos_node(const BndArgs &a, long long b) {
   const Grid &g = a.g;
   const long long np = g.np;
   if (b >= a.nb) return;
@@ -391,6 +392,23 @@ __global__ __launch_bounds__(256) void seam_eos_kernel(BndArgs a) {
     for (int q = 0; q < 5; q++) a.qo[n + q * np] = qn[q];
   }
   a.po[n] = (a.gamma - 1) * (qn[4] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) / qn[0]);
+}
+
+__global__ __launch_bounds__(256) void seam_eos_kernel(BndArgs a) {
+  seam_eos_node(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// batched seams + EOS over several grids: bstart[r] is the first workgroup of
+// grid r (256 boundary nodes per workgroup)
+__global__ __launch_bounds__(256) void seam_batch_kernel(const BndArgs *__restrict__ tab, const int *__restrict__ bstart,
+                                                         int nreq) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = nreq;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bstart[mid] <= b) lo = mid; else hi = mid;
+  }
+  seam_eos_node(tab[lo], (long long)(b - bstart[lo]) * 256 + threadIdx.x);
 }
 
 // periodic seams after the interpolation of the immersed body: Q and P both
@@ -478,6 +496,35 @@ int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const
   hipLaunchKernelGGL(seam_eos_kernel, grid(a.nb), dim3(bs), 0, c->stream, a);
   CLES_HIP(c, hipGetLastError());
   return CLES_OK;
+}
+
+size_t seam_args_bytes() { return sizeof(BndArgs); }
+
+// arguments of the seam + EOS pass of a fully periodic grid; returns the
+// number of 256-node workgroups it needs
+int seam_batch_args(cles_ctx *c, int in, int out, void *dst) {
+  const cles_params &p = c->p;
+  BndArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = c->g;
+  a.q = c->Q[in];
+  a.p = c->P[in];
+  a.qold = c->Q[c->cur];
+  a.qo = c->Q[out];
+  a.po = c->P[out];
+  a.aj = c->aj; a.eta = c->eta; a.zet = c->zet; a.nv = c->nvert; a.coor = c->coor;
+  for (int q = 0; q < 6; q++) a.bc[q] = p.bctype[q];
+  a.gamma = p.gamma; a.Ma = p.Ma; a.Twall = p.Twall; a.dt = p.dt;
+  a.periodic = 1;
+  const long long mx = c->g.mx, my = c->g.my, mz = c->g.mz;
+  a.nb = 2 * mx * my + 2 * mx * (mz - 2) + 2 * (my - 2) * (mz - 2);
+  memcpy(dst, &a, sizeof(a));
+  return (int)((a.nb + 255) / 256);
+}
+
+hipError_t launch_seam_batch(const void *tab, const int *bstart, int nreq, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(seam_batch_kernel, dim3(nblocks), dim3(256), 0, s, (const BndArgs *)tab, bstart, nreq);
+  return hipGetLastError();
 }
 
 int launch_boundary_post(cles_ctx *c, int buf) {

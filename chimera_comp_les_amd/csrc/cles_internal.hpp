@@ -182,6 +182,8 @@ struct cles_ctx {
   double *aj, *nvert;       // 1 plane each
   double *fil2;             // (pow(1/aj, 1/3))^2: squared LES filter width per node
   double *tfil2;            // squared test-filter width (2 or 5^(1/3) filter widths)
+  double ref_rho, ref_p;    // FP32-bulk reference state (node (1,1,1) of the uploaded state)
+  int tile;                 // stage-kernel tile shape (0: 32x4x2, 1: 16x4x4, 2: 16x8x2, 3: 64x2x2)
   int has_ib;               // any nvert != 0
   int immersed;             // a surface was classified: RK steps end with the interpolation
   cles_ibm_dev *ibm;
@@ -218,6 +220,16 @@ int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const
 // bctype b is implemented on side q (0..5 = i-, i+, j-, j+, k-, k+)
 bool supported_bc(int side, int b);
 int launch_les(cles_ctx *c);
+// batched stages of several periodic grids (serving)
+size_t stage_args_bytes();
+int stage_batch_args(cles_ctx *c, int in, int out, int istage, const double coef[3], int tile, void *dst);
+hipError_t launch_stage_batch(int tile, int fp32, int visc, const void *tab, const int *tstart, int nreq, int nblocks,
+                              hipStream_t s);
+size_t seam_args_bytes();
+int seam_batch_args(cles_ctx *c, int in, int out, void *dst);
+hipError_t launch_seam_batch(const void *tab, const int *bstart, int nreq, int nblocks, hipStream_t s);
+int rk_coef(const cles_params &p, int istage, double coef[3]);
+size_t ctx_footprint(const cles_params *p);
 // immersed body
 int launch_ibm_interp(cles_ctx *c, int buf, int qold_buf, int add_dundt);
 int launch_snapshot(cles_ctx *c, int buf);

@@ -19,19 +19,33 @@
 // per workgroup (1.26 faces per node-face on average) instead of twice per
 // node.  Tiles are numbered so that each XCD (blockIdx % 8) receives a
 // contiguous slab of tiles and keeps its neighbours in its own L2.
+#include <cstring>
+#include <type_traits>
+
 #include "cles_internal.hpp"
 
 namespace cles {
 
 namespace {
 
-constexpr int TX = 32, TY = 4, TZ = 2, NT = TX * TY * TZ;
-constexpr int NFI = (TX + 1) * TY * TZ;
-constexpr int NFJ = TX * (TY + 1) * TZ;
-constexpr int NFK = TX * TY * (TZ + 1);
-constexpr int NF = NFI + NFJ + NFK;
-constexpr int WX = TX + 2, WY = TY + 2, WZ = TZ + 2, NW = WX * WY * WZ;
+constexpr int NT = 256;    // threads per workgroup, one interior node each
 constexpr int NSLOT = 64;  // spread of the inf-norm atomics
+
+// Tile of interior nodes owned by one workgroup, with the face counts of its
+// convective/viscous face buffer and its primitive window (one-node halo).
+template <int X, int Y, int Z>
+struct Tile {
+  static_assert(X * Y * Z == NT, "one node per thread");
+  static constexpr int TX = X, TY = Y, TZ = Z;
+  static constexpr int NFI = (X + 1) * Y * Z, NFJ = X * (Y + 1) * Z, NFK = X * Y * (Z + 1);
+  static constexpr int NF = NFI + NFJ + NFK;
+  static constexpr int WX = X + 2, WY = Y + 2, WZ = Z + 2, NW = WX * WY * WZ;
+};
+// tile shapes the launcher can pick (index = cles_ctx::tile)
+typedef Tile<32, 4, 2> TileA;
+typedef Tile<16, 4, 4> TileB;
+typedef Tile<16, 8, 2> TileC;
+typedef Tile<64, 2, 2> TileD;
 
 struct StageArgs {
   Grid g;
@@ -43,6 +57,9 @@ struct StageArgs {
   int bc[6];
   int fluxorder, eno, central, iso, has_ib;
   double ren, Ma, Pr, gamma, Tref, ratio, dt;
+  // reference state of the FP32-bulk form: the float arithmetic carries the
+  // deviations from it, the reference parts of the fluxes stay in double
+  double rho_ref, p_ref, e_ref, h_ref, t_ref;
   int suth;
   SigmaSpec sig;
   double a_rhs, a_old, a_cur;
@@ -121,16 +138,95 @@ __device__ __forceinline__ void face_metric(const StageArgs &a, int d, long long
   h[2] = T(0.5 * (m[n0 + 2 * np] + m[n1 + 2 * np]));
 }
 
+// 1/rho to ~1e-15 relative without a double division: float reciprocal and
+// one Newton step in double
+__device__ __forceinline__ double inv_refined(double r) {
+  double y = (double)(1.0f / (float)r);
+  return fma(y, fma(-r, y, 1.0), y);
+}
+__device__ __forceinline__ double ucon_d(const double *s, const double *h) {
+  const double inv = inv_refined(s[0]);
+  return h[0] * (s[1] * inv) + h[1] * (s[2] * inv) + h[2] * (s[3] * inv);
+}
+
+// FP32-bulk Rusanov flux on double face states: the float arithmetic sees
+// the deviations rho - rho_ref, E - e_ref, p - p_ref and the state jumps;
+// rho_ref and e_ref + p_ref = h_ref times the mean contravariant velocity and
+// p_ref times the face metric (the GCL part) are added in double.  Same
+// algebra as rusanov() (rhs.c:2596-2640).
+__device__ void rusanov_split(const StageArgs &a, const double *L, const double *R, const double *hd, double uconL,
+                              double uconR, double F[5]) {
+  const float h0 = (float)hd[0], h1 = (float)hd[1], h2 = (float)hd[2];
+  const float gii = h0 * h0 + h1 * h1 + h2 * h2;
+  const float CsoundL = 1.4f * (float)L[5] / (float)L[0];
+  const float CsoundR = 1.4f * (float)R[5] / (float)R[0];
+  const float uL = (float)uconL, uR = (float)uconR;
+  const float saL = fabsf(uL) + sqrtf(fabsf(CsoundL) * gii);
+  const float saR = fabsf(uR) + sqrtf(fabsf(CsoundR) * gii);
+  const float SA = saL < saR ? saR : saL;
+  const float rL = (float)(L[0] - a.rho_ref), rR = (float)(R[0] - a.rho_ref);
+  const float pL = (float)(L[5] - a.p_ref), pR = (float)(R[5] - a.p_ref);
+  const float eL = (float)(L[4] - a.e_ref), eR = (float)(R[4] - a.e_ref);
+  float jump[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) jump[q] = (float)(R[q] - L[q]);
+  const double um = 0.5 * (uconL + uconR);
+  F[0] = a.rho_ref * um + (double)(0.5f * (rL * uL + rR * uR) - 0.5f * SA * jump[0]);
+  const float hh[3] = {h0, h1, h2};
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    F[1 + c] = a.p_ref * hd[c] +
+               (double)(0.5f * (((float)L[1 + c] * uL + (float)R[1 + c] * uR) + (pL * hh[c] + pR * hh[c])) -
+                        0.5f * SA * jump[1 + c]);
+  F[4] = a.h_ref * um + (double)(0.5f * ((pL + eL) * uL + (pR + eR) * uR) - 0.5f * SA * jump[4]);
+}
+
+__device__ __forceinline__ void load6d(const StageArgs &a, long long n, double s[6]) { load6<double>(a, n, s); }
+
+// FP32-bulk reconstruction: WENO / ENO / MUSCL are equivariant under a common
+// shift of their three inputs, so they run in float on the deviations from
+// the node n0 value and the shift is added back in double.
+__device__ void recon_split(const StageArgs &a, bool muscl, long long nm, long long n0, long long n1, long long n2,
+                            bool eno_left, int eno_mask, double L[6], double R[6]) {
+  double sm[6], s0[6], s1[6], s2[6];
+  load6d(a, n0, s0);
+  load6d(a, n1, s1);
+  if (muscl) {
+    if (nvat(a, nm) > 2.9) load6d(a, n1, sm); else load6d(a, nm, sm);
+    if (nvat(a, n2) > 2.9) load6d(a, n0, s2); else load6d(a, n2, s2);
+  } else {
+    load6d(a, nm, sm);
+    load6d(a, n2, s2);
+  }
+  const bool solid2 = nvat(a, n2) > 2.9;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const float dm = (float)(sm[q] - s0[q]), d1 = (float)(s1[q] - s0[q]), d2 = (float)(s2[q] - s0[q]);
+    float l, r;
+    if (muscl) {
+      l = muscl_ql(dm, 0.0f, d1);
+      r = muscl_qr(0.0f, d1, d2);
+    } else {
+      l = (eno_left || (eno_mask >> q & 1)) ? eno_ql(0.0f, d1, d2, a.eno) : weno_ql(dm, 0.0f, d1);
+      r = solid2 ? eno_qr(dm, 0.0f, d1, a.eno) : weno_qr(0.0f, d1, d2);
+    }
+    L[q] = s0[q] + (double)l;
+    R[q] = s0[q] + (double)r;
+  }
+}
+
 // formConvection face (rhs.c:2646-2814, 2817-2971, 2973-3121)
 template <typename T>
-__device__ void conv_face_plain(const StageArgs &a, int d, int k, int j, int i, T F[5]) {
+__device__ void conv_face_plain(const StageArgs &a, int d, int k, int j, int i, double F[5]) {
   const Grid &g = a.g;
   const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
   const int id = d == 0 ? i : d == 1 ? j : k;
   const long long s = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
   const long long n0 = nid(g, k, j, i), n1 = n0 + s;
   const bool iso = a.iso != 0;
-  T L[6], R[6];
+  constexpr bool F32 = sizeof(T) == 4;
+  typedef typename std::conditional<F32, double, T>::type S;  // face-state type
+  S L[6], R[6];
   if (d == 1 && j == 0 && iso && a.bc[2] == 1) {
     load6(a, n0, L);
     load6(a, n0, R);
@@ -142,26 +238,110 @@ __device__ void conv_face_plain(const StageArgs &a, int d, int k, int j, int i, 
     load6(a, n1, R);
   } else {
     const long long nm = n0 - s, n2 = n0 + 2 * s;
-    if (a.fluxorder == 2 || (d != 0 && j < 30 && k < 7)) {
-      recon_muscl(a, nm, n0, n1, n2, L, R);
-    } else {
-      bool eno_left = nvat(a, nm) > 2.9;
-      int mask = 0;
+    const bool muscl = a.fluxorder == 2 || (d != 0 && j < 30 && k < 7);
+    bool eno_left = false;
+    int mask = 0;
+    if (!muscl) {
+      eno_left = nvat(a, nm) > 2.9;
       if (d == 0) eno_left = eno_left && (i + 2) < g.mx;
       if (d == 1 && !eno_left && j == 1 && iso && a.bc[2] == 1) mask = (1 << 0) | (1 << 4) | (1 << 5);
-      recon_weno(a, nm, n0, n1, n2, eno_left, mask, L, R);
+    }
+    if constexpr (F32) {
+      recon_split(a, muscl, nm, n0, n1, n2, eno_left, mask, L, R);
+    } else {
+      if (muscl) recon_muscl(a, nm, n0, n1, n2, L, R);
+      else recon_weno(a, nm, n0, n1, n2, eno_left, mask, L, R);
     }
   }
-  T h[3];
+  S h[3];
   face_metric(a, d, n0, n1, h);
-  T uconL = (id == m - 2 && a.bc[2 * d + 1] == 1) ? ucon_of(R, h) : ucon_of(L, h);
-  T uconR = (id == 0 && a.bc[2 * d] == 1) ? uconL : ucon_of(R, h);
-  rusanov(L, R, h, uconL, uconR, F);
+  if constexpr (F32) {
+    const double uconL = (id == m - 2 && a.bc[2 * d + 1] == 1) ? ucon_d(R, h) : ucon_d(L, h);
+    const double uconR = (id == 0 && a.bc[2 * d] == 1) ? uconL : ucon_d(R, h);
+    rusanov_split(a, L, R, h, uconL, uconR, F);
+  } else {
+    T uconL = (id == m - 2 && a.bc[2 * d + 1] == 1) ? ucon_of(R, h) : ucon_of(L, h);
+    T uconR = (id == 0 && a.bc[2 * d] == 1) ? uconL : ucon_of(R, h);
+    T Ft[5];
+    rusanov(L, R, h, uconL, uconR, Ft);
+#pragma unroll
+    for (int q = 0; q < 5; q++) F[q] = Ft[q];
+  }
 }
 
-// formConvectionLES face (rhs.c:3466-3706, 3714-3937, 3947-4168)
 template <typename T>
-__device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, T F[5]) {
+__device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, double F[5]);
+
+// FP32-bulk four-point central flux (sigma = 1).  fp_2(c, U) = c * fp4(U) for
+// a constant c, so rho, rhoE + p and the pressure are split into reference
+// parts, whose fluxes rho_ref fp4(U), h_ref fp4(U) and p_ref h use the
+// contravariant velocity in double, and float deviations for the rest.
+__device__ bool conv_face_les_split(const StageArgs &a, int d, int k, int j, int i, double F[5]) {
+  const Grid &g = a.g;
+  const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
+  const int id = d == 0 ? i : d == 1 ? j : k;
+  if (!(id > 0 && id < m - 2 && (a.central == 2 || a.central == 3))) return false;
+  if (sigma_at(a, d, k, j, i) != 1.0) return false;
+  const long long s = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
+  const long long n0 = nid(g, k, j, i), n1 = n0 + s, nm = n0 - s, n2 = n0 + 2 * s;
+  const double sum = d == 0 ? nvat(a, n0) + nvat(a, n1) + nvat(a, n2) + nvat(a, nm)
+                            : nvat(a, nm) + nvat(a, n0) + nvat(a, n1) + nvat(a, n2);
+  if (!(sum < 2.9)) return false;
+  const long long np = g.np;
+  const double *mt = metric(a, d);
+  double hd[3];
+  face_metric(a, d, n0, n1, hd);
+  float r[4], u[4], v[4], w[4], ep[4], pp[4], U[4];
+  double Ud[4];
+  const long long nn[4] = {nm, n0, n1, n2};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const long long n = nn[q];
+    const double rd = a.q[n], ru = a.q[n + np], rv = a.q[n + 2 * np], rw = a.q[n + 3 * np];
+    const double ed = a.q[n + 4 * np], pd = a.p[n];
+    Ud[q] = (mt[n] * ru + mt[n + np] * rv + mt[n + 2 * np] * rw) * inv_refined(rd);
+    U[q] = (float)Ud[q];
+    r[q] = (float)(rd - a.rho_ref);
+    u[q] = (float)ru;
+    v[q] = (float)rv;
+    w[q] = (float)rw;
+    pp[q] = (float)(pd - a.p_ref);
+    ep[q] = (float)((ed + pd) - a.h_ref);
+  }
+  const float h0 = (float)hd[0], h1 = (float)hd[1], h2 = (float)hd[2];
+  float Fc[5];
+  if (a.central == 2) {
+    Fc[0] = fp_2(r, U);
+    Fc[1] = fp_2(u, U) + fp4(pp) * h0;
+    Fc[2] = fp_2(v, U) + fp4(pp) * h1;
+    Fc[3] = fp_2(w, U) + fp4(pp) * h2;
+    Fc[4] = fp_2(ep, U);
+  } else {
+    Fc[0] = fp_1(r, U);
+    Fc[1] = fp_1(u, U) + fp4(pp) * h0;
+    Fc[2] = fp_1(v, U) + fp4(pp) * h1;
+    Fc[3] = fp_1(w, U) + fp4(pp) * h2;
+    Fc[4] = fp_1(ep, U);
+  }
+  const double fu = fp4(Ud);
+  F[0] = a.rho_ref * fu + (double)Fc[0];
+  F[1] = a.p_ref * hd[0] + (double)Fc[1];
+  F[2] = a.p_ref * hd[1] + (double)Fc[2];
+  F[3] = a.p_ref * hd[2] + (double)Fc[3];
+  F[4] = a.h_ref * fu + (double)Fc[4];
+  return true;
+}
+
+// formConvectionLES face (rhs.c:3466-3706, 3714-3937, 3947-4168).  The
+// FP32-bulk form uses the split central flux above and, for the rare faces
+// it does not cover (boundaries, immersed nodes, sigma < 1), this function in
+// double.
+template <typename T>
+__device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, double F[5]) {
+  if constexpr (sizeof(T) == 4) {
+    if (!conv_face_les_split(a, d, k, j, i, F)) conv_face_les<double>(a, d, k, j, i, F);
+    return;
+  }
   const Grid &g = a.g;
   const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
   const int id = d == 0 ? i : d == 1 ? j : k;
@@ -254,19 +434,24 @@ __device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, T 
   for (int q = 0; q < 5; q++) F[q] = Fw[q] * (T(1.0) - sigma) + Fc[q] * sigma;
 }
 
+// instantiate the double version used by the float fallback
+template __device__ void conv_face_les<double>(const StageArgs &, int, int, int, int, double *);
+
 // ------------------------------ viscous ----------------------------------
 struct Win {  // LDS window of primitives with a one-node halo
   int i0, j0, k0;  // global coordinates of window node (0,0,0)
 };
+template <class TL>
 __device__ __forceinline__ int widx(const Win &w, int k, int j, int i) {
-  return ((k - w.k0) * WY + (j - w.j0)) * WX + (i - w.i0);
+  return ((k - w.k0) * TL::WY + (j - w.j0)) * TL::WX + (i - w.i0);
 }
 __device__ __forceinline__ bool ibn(double nv) { return nv > 0.5 && nv < 7.; }
 
 // one-sided cross derivatives next to immersed-boundary rows (rhs.c:1498-1575)
-template <typename T>
+template <class TL, typename T>
 __device__ __forceinline__ void cross_deriv(const StageArgs &a, const T *W, int wA, int wB, int wt,
                                             long long gA, long long gB, long long gt, bool q8, T du[3], T &dT) {
+  constexpr int NW = TL::NW;
   const T *U = W, *V = W + NW, *Wz = W + 2 * NW, *Tm = W + 3 * NW;
   int mode = 2;
   if (a.has_ib) {
@@ -294,15 +479,16 @@ __device__ __forceinline__ void cross_deriv(const StageArgs &a, const T *W, int 
 
 // formViscous face (rhs.c:1487-1707, 1719-1918, 1926-2121); G[0..3] =
 // rhoU..rhoE flux, al = LES stress (alpha) of the face.
-template <typename T, bool LES>
+template <class TL, typename T, bool LES>
 __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d, int k, int j, int i, T G[4],
                           T al[3]) {
+  constexpr int WX = TL::WX, WY = TL::WY, NW = TL::NW;
   const Grid &g = a.g;
   const long long np = g.np;
   const long long gs[3] = {1, g.sy, g.sz};
   const int ws[3] = {1, WX, WX * WY};
   const long long gA = nid(g, k, j, i), gB = gA + gs[d];
-  const int wA = widx(win, k, j, i), wB = wA + ws[d];
+  const int wA = widx<TL>(win, k, j, i), wB = wA + ws[d];
   const T *U = W, *V = W + NW, *Wz = W + 2 * NW, *Tm = W + 3 * NW;
   T dd[3][3], dT[3];
   dd[d][0] = U[wB] - U[wA];
@@ -312,7 +498,7 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
 #pragma unroll
   for (int t = 0; t < 3; t++) {
     if (t == d) continue;
-    cross_deriv(a, W, wA, wB, ws[t], gA, gB, gs[t], d == 0 && t == 2, dd[t], dT[t]);
+    cross_deriv<TL>(a, W, wA, wB, ws[t], gA, gB, gs[t], d == 0 && t == 2, dd[t], dT[t]);
   }
   const double *cs = a.csi, *et = a.eta, *ze = a.zet;
   T csi0 = T(0.5 * (cs[gA] + cs[gB])), csi1 = T(0.5 * (cs[gA + np] + cs[gB + np])),
@@ -428,7 +614,9 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // face decode: f in [0, NF) -> direction and local coordinates
+template <class TL>
 __device__ __forceinline__ void decode_face(int f, int &d, int &lx, int &ly, int &lz) {
+  constexpr int TX = TL::TX, TY = TL::TY, NFI = TL::NFI, NFJ = TL::NFJ;
   if (f < NFI) {
     d = 0; lx = f % (TX + 1); ly = (f / (TX + 1)) % TY; lz = f / ((TX + 1) * TY);
   } else if (f < NFI + NFJ) {
@@ -439,25 +627,24 @@ __device__ __forceinline__ void decode_face(int f, int &d, int &lx, int &ly, int
     d = 2; lx = f % TX; ly = (f / TX) % TY; lz = f / (TX * TY);
   }
 }
-__device__ __forceinline__ int fi_of(int lx, int ly, int lz) { return (lz * TY + ly) * (TX + 1) + lx; }
-__device__ __forceinline__ int fj_of(int lx, int ly, int lz) { return NFI + (lz * (TY + 1) + ly) * TX + lx; }
-__device__ __forceinline__ int fk_of(int lx, int ly, int lz) { return NFI + NFJ + (lz * TY + ly) * TX + lx; }
+template <class TL> __device__ __forceinline__ int fi_of(int lx, int ly, int lz) {
+  return (lz * TL::TY + ly) * (TL::TX + 1) + lx;
+}
+template <class TL> __device__ __forceinline__ int fj_of(int lx, int ly, int lz) {
+  return TL::NFI + (lz * (TL::TY + 1) + ly) * TL::TX + lx;
+}
+template <class TL> __device__ __forceinline__ int fk_of(int lx, int ly, int lz) {
+  return TL::NFI + TL::NFJ + (lz * TL::TY + ly) * TL::TX + lx;
+}
 
-template <typename T, int NC>
-struct StageSmem {
-  T F[NC][NF];
-};
-
-template <typename T, bool LES, bool VISC, int MODE>
-__global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
+// One workgroup's tile of one grid; b only spreads the norm atomics.
+template <class TL, typename T, bool LES, bool VISC, int MODE>
+__device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) {
+  constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, NF = TL::NF, WX = TL::WX, WY = TL::WY, NW = TL::NW;
   constexpr int FC = VISC ? (LES ? 7 : 5) : 5;
-  __shared__ T Fs[FC][NF];
+  __shared__ double Fs[FC][NF];
   __shared__ T Ws[VISC ? 4 * NW : 1];
   const Grid &g = a.g;
-  // XCD-aware tile numbering: blockIdx % 8 selects the XCD, each XCD walks a
-  // contiguous range of tiles.
-  const int b = blockIdx.x;
-  const int tile = (b & 7) * a.per_xcd + (b >> 3);
   if (tile >= a.ntx * a.nty * a.ntz) return;
   const int tix = tile % a.ntx, tiy = (tile / a.ntx) % a.nty, tiz = tile / (a.ntx * a.nty);
   const int i0 = 1 + tix * TX, j0 = 1 + tiy * TY, k0 = 1 + tiz * TZ;
@@ -467,13 +654,13 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
   // ---- phase A: convective face fluxes -> LDS
   for (int f = t; f < NF; f += NT) {
     int d, lx, ly, lz;
-    decode_face(f, d, lx, ly, lz);
+    decode_face<TL>(f, d, lx, ly, lz);
     int gi = i0 + lx - (d == 0), gj = j0 + ly - (d == 1), gk = k0 + lz - (d == 2);
     bool ok = d == 0 ? (gi <= g.mx - 2 && gj <= g.my - 2 && gk <= g.mz - 2)
                      : d == 1 ? (gj <= g.my - 2 && gi <= g.mx - 2 && gk <= g.mz - 2)
                               : (gk <= g.mz - 2 && gi <= g.mx - 2 && gj <= g.my - 2);
     if (!ok) continue;
-    T F[5];
+    double F[5];
     if (LES) conv_face_les<T>(a, d, gk, gj, gi, F);
     else conv_face_plain<T>(a, d, gk, gj, gi, F);
 #pragma unroll
@@ -488,9 +675,9 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
   const long long n = valid ? nid(g, gk, gj, gi) : 0;
   double conv[5];
   {
-    const int a0 = fi_of(lx, ly, lz), a1 = fi_of(lx + 1, ly, lz);
-    const int b0 = fj_of(lx, ly, lz), b1 = fj_of(lx, ly + 1, lz);
-    const int c0 = fk_of(lx, ly, lz), c1 = fk_of(lx, ly, lz + 1);
+    const int a0 = fi_of<TL>(lx, ly, lz), a1 = fi_of<TL>(lx + 1, ly, lz);
+    const int b0 = fj_of<TL>(lx, ly, lz), b1 = fj_of<TL>(lx, ly + 1, lz);
+    const int c0 = fk_of<TL>(lx, ly, lz), c1 = fk_of<TL>(lx, ly, lz + 1);
 #pragma unroll
     for (int q = 0; q < 5; q++)
       conv[q] = double(Fs[q][a1] - Fs[q][a0] + Fs[q][b1] - Fs[q][b0] + Fs[q][c1] - Fs[q][c0]);
@@ -513,18 +700,18 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
       Ws[w] = T(a.q[m + np] / r);
       Ws[NW + w] = T(a.q[m + 2 * np] / r);
       Ws[2 * NW + w] = T(a.q[m + 3 * np] / r);
-      Ws[3 * NW + w] = T(a.p[m] / r);
+      Ws[3 * NW + w] = T(a.p[m] / r - a.t_ref);  // t_ref = 0 in FP64: only gradients of p/rho are used
     }
     __syncthreads();
     // ---- phase C2: viscous face fluxes -> LDS
     for (int f = t; f < NF; f += NT) {
       int d, fx, fy, fz;
-      decode_face(f, d, fx, fy, fz);
+      decode_face<TL>(f, d, fx, fy, fz);
       int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
       bool ok = fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2;
       if (!ok) continue;
       T G[4], al[3] = {T(0), T(0), T(0)};
-      visc_face<T, LES>(a, Ws, win, d, fk, fj, fi, G, al);
+      visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, G, al);
 #pragma unroll
       for (int q = 0; q < 4; q++) Fs[q][f] = G[q];
       if (LES) {
@@ -534,9 +721,9 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
     }
     __syncthreads();
     // ---- phase D1: viscous residual (+ LES energy term, rhs.c:2153-2197)
-    const int a0 = fi_of(lx, ly, lz), a1 = fi_of(lx + 1, ly, lz);
-    const int b0 = fj_of(lx, ly, lz), b1 = fj_of(lx, ly + 1, lz);
-    const int c0 = fk_of(lx, ly, lz), c1 = fk_of(lx, ly, lz + 1);
+    const int a0 = fi_of<TL>(lx, ly, lz), a1 = fi_of<TL>(lx + 1, ly, lz);
+    const int b0 = fj_of<TL>(lx, ly, lz), b1 = fj_of<TL>(lx, ly + 1, lz);
+    const int c0 = fk_of<TL>(lx, ly, lz), c1 = fk_of<TL>(lx, ly, lz + 1);
 #pragma unroll
     for (int q = 0; q < 4; q++)
       visc[q + 1] = double(Fs[q][a1] - Fs[q][a0] + Fs[q][b1] - Fs[q][b0] + Fs[q][c1] - Fs[q][c0]);
@@ -550,7 +737,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
       T e = u0 * (rR * Fs[4][a1] - rL * Fs[4][a0] + rU * Fs[4][b1] - rD * Fs[4][b0] + rF * Fs[4][c1] - rB * Fs[4][c0]) +
             u1 * (rR * Fs[5][a1] - rL * Fs[5][a0] + rU * Fs[5][b1] - rD * Fs[5][b0] + rF * Fs[5][c1] - rB * Fs[5][c0]) +
             u2 * (rR * Fs[6][a1] - rL * Fs[6][a0] + rU * Fs[6][b1] - rD * Fs[6][b0] + rF * Fs[6][c1] - rB * Fs[6][c0]);
-      visc[4] = double(T(visc[4]) + e);
+      visc[4] = visc[4] + double(e);
     }
   }
   if (!valid) return;
@@ -602,20 +789,79 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
   a.po[n] = (a.gamma - 1) * (qn[4] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) / qn[0]);
 }
 
-template <typename T, bool LES, bool VISC>
+template <class TL, typename T, bool LES, bool VISC, int MODE>
+__global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
+  // XCD-aware tile numbering: blockIdx % 8 selects the XCD, each XCD walks a
+  // contiguous range of tiles and keeps their shared halos in its own L2.
+  const int b = blockIdx.x;
+  stage_tile<TL, T, LES, VISC, MODE>(a, (b & 7) * a.per_xcd + (b >> 3), b);
+}
+
+// Batched stage: the workgroups of one launch cover the tiles of several
+// independent grids; tstart[r] is the first workgroup of grid r (prefix sum
+// of the tile counts, tstart[nreq] = total).
+template <class TL, typename T, bool LES, bool VISC, int MODE>
+__global__ __launch_bounds__(NT) void stage_batch_kernel(const StageArgs *__restrict__ tab, const int *__restrict__ tstart,
+                                                         int nreq) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = nreq;  // largest r with tstart[r] <= b
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tstart[mid] <= b) lo = mid; else hi = mid;
+  }
+  stage_tile<TL, T, LES, VISC, MODE>(tab[lo], b - tstart[lo], b);
+}
+
+template <class TL>
+void tile_counts(const Grid &g, StageArgs &a) {
+  a.ntx = (g.mx - 2 + TL::TX - 1) / TL::TX;
+  a.nty = (g.my - 2 + TL::TY - 1) / TL::TY;
+  a.ntz = (g.mz - 2 + TL::TZ - 1) / TL::TZ;
+  a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
+}
+
+template <class TL, typename T, bool LES, bool VISC>
 hipError_t launch_t(const StageArgs &a, int mode, int nblocks, hipStream_t s) {
   if (mode == 0)
-    hipLaunchKernelGGL((stage_kernel<T, LES, VISC, 0>), dim3(nblocks), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 0>), dim3(nblocks), dim3(NT), 0, s, a);
   else
-    hipLaunchKernelGGL((stage_kernel<T, LES, VISC, 1>), dim3(nblocks), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 1>), dim3(nblocks), dim3(NT), 0, s, a);
   return hipGetLastError();
+}
+
+// every (precision, LES, viscous) variant of one tile shape
+template <class TL>
+hipError_t launch_shape(const cles_params &p, StageArgs &a, int mode, hipStream_t s) {
+  tile_counts<TL>(a.g, a);
+  const int nblocks = a.per_xcd * 8;
+  const bool les = p.les != 0, visc = !p.invicid;
+  if (p.precision == CLES_PREC_FP32BULK) {
+    if (les) return visc ? launch_t<TL, float, true, true>(a, mode, nblocks, s) : launch_t<TL, float, true, false>(a, mode, nblocks, s);
+    return visc ? launch_t<TL, float, false, true>(a, mode, nblocks, s) : launch_t<TL, float, false, false>(a, mode, nblocks, s);
+  }
+  if (les) return visc ? launch_t<TL, double, true, true>(a, mode, nblocks, s) : launch_t<TL, double, true, false>(a, mode, nblocks, s);
+  return visc ? launch_t<TL, double, false, true>(a, mode, nblocks, s) : launch_t<TL, double, false, false>(a, mode, nblocks, s);
+}
+
+template <class TL, typename T, bool VISC>
+hipError_t launch_batch_t(const StageArgs *tab, const int *tstart, int nreq, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((stage_batch_kernel<TL, T, false, VISC, 1>), dim3(nblocks), dim3(NT), 0, s, tab, tstart, nreq);
+  return hipGetLastError();
+}
+
+template <class TL>
+hipError_t launch_batch_shape(int fp32, int visc, const StageArgs *tab, const int *tstart, int nreq, int nblocks,
+                              hipStream_t s) {
+  if (fp32) return visc ? launch_batch_t<TL, float, true>(tab, tstart, nreq, nblocks, s)
+                        : launch_batch_t<TL, float, false>(tab, tstart, nreq, nblocks, s);
+  return visc ? launch_batch_t<TL, double, true>(tab, tstart, nreq, nblocks, s)
+              : launch_batch_t<TL, double, false>(tab, tstart, nreq, nblocks, s);
 }
 
 }  // namespace
 
-int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const double coef[3]) {
+void fill_stage_args(cles_ctx *c, int in, int out, int istage, const double coef[3], StageArgs &a) {
   const cles_params &p = c->p;
-  StageArgs a;
   a.g = c->g;
   a.q = c->Q[in];
   a.p = c->P[in];
@@ -629,25 +875,55 @@ int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const doubl
   a.fluxorder = p.fluxorder; a.eno = p.eno_order; a.central = p.central; a.iso = p.isothermal;
   a.has_ib = c->has_ib;
   a.ren = p.ren; a.Ma = p.Ma; a.Pr = p.Pr; a.gamma = p.gamma; a.Tref = p.Tref; a.ratio = p.ratio; a.dt = p.dt;
+  const bool f32 = p.precision == CLES_PREC_FP32BULK;
+  a.rho_ref = f32 ? c->ref_rho : 0.0;
+  a.p_ref = f32 ? c->ref_p : 0.0;
+  a.e_ref = a.p_ref / (p.gamma - 1);
+  a.h_ref = a.e_ref + a.p_ref;
+  a.t_ref = f32 ? c->ref_p / c->ref_rho : 0.0;
   a.suth = p.sutherland;
   a.sig = c->sig;
   a.a_rhs = coef[0]; a.a_old = coef[1]; a.a_cur = coef[2];
   a.istage = istage;
-  a.ntx = (c->g.mx - 2 + TX - 1) / TX;
-  a.nty = (c->g.my - 2 + TY - 1) / TY;
-  a.ntz = (c->g.mz - 2 + TZ - 1) / TZ;
-  const int ntiles = a.ntx * a.nty * a.ntz;
-  a.per_xcd = (ntiles + 7) / 8;
   a.norms = c->dnorm;
-  const int nblocks = a.per_xcd * 8;
-  const bool les = p.les != 0, visc = !p.invicid;
+}
+
+size_t stage_args_bytes() { return sizeof(StageArgs); }
+
+int stage_batch_args(cles_ctx *c, int in, int out, int istage, const double coef[3], int tile, void *dst) {
+  StageArgs a;
+  fill_stage_args(c, in, out, istage, coef, a);
+  a.norms = nullptr;
+  switch (tile) {
+    case 1: tile_counts<TileB>(a.g, a); break;
+    case 2: tile_counts<TileC>(a.g, a); break;
+    case 3: tile_counts<TileD>(a.g, a); break;
+    default: tile_counts<TileA>(a.g, a); break;
+  }
+  memcpy(dst, &a, sizeof(a));
+  return a.ntx * a.nty * a.ntz;
+}
+
+hipError_t launch_stage_batch(int tile, int fp32, int visc, const void *tab, const int *tstart, int nreq, int nblocks,
+                              hipStream_t s) {
+  const StageArgs *t = (const StageArgs *)tab;
+  switch (tile) {
+    case 1: return launch_batch_shape<TileB>(fp32, visc, t, tstart, nreq, nblocks, s);
+    case 2: return launch_batch_shape<TileC>(fp32, visc, t, tstart, nreq, nblocks, s);
+    case 3: return launch_batch_shape<TileD>(fp32, visc, t, tstart, nreq, nblocks, s);
+    default: return launch_batch_shape<TileA>(fp32, visc, t, tstart, nreq, nblocks, s);
+  }
+}
+
+int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const double coef[3]) {
+  StageArgs a;
+  fill_stage_args(c, in, out, istage, coef, a);
   hipError_t e;
-  if (p.precision == CLES_PREC_FP32BULK) {
-    if (les) e = visc ? launch_t<float, true, true>(a, mode, nblocks, c->stream) : launch_t<float, true, false>(a, mode, nblocks, c->stream);
-    else e = visc ? launch_t<float, false, true>(a, mode, nblocks, c->stream) : launch_t<float, false, false>(a, mode, nblocks, c->stream);
-  } else {
-    if (les) e = visc ? launch_t<double, true, true>(a, mode, nblocks, c->stream) : launch_t<double, true, false>(a, mode, nblocks, c->stream);
-    else e = visc ? launch_t<double, false, true>(a, mode, nblocks, c->stream) : launch_t<double, false, false>(a, mode, nblocks, c->stream);
+  switch (c->tile) {
+    case 1: e = launch_shape<TileB>(c->p, a, mode, c->stream); break;
+    case 2: e = launch_shape<TileC>(c->p, a, mode, c->stream); break;
+    case 3: e = launch_shape<TileD>(c->p, a, mode, c->stream); break;
+    default: e = launch_shape<TileA>(c->p, a, mode, c->stream); break;
   }
   CLES_HIP(c, e);
   return CLES_OK;

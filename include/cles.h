@@ -174,6 +174,49 @@ int cles_time_stage_kernel(cles_ctx *c, int nrep, double *ms);
  * recomputed from the current state, nothing else changes. */
 int cles_time_les(cles_ctx *c, int nrep, double *ms);
 
+/* Domain integrals of the current state over the unique interior nodes,
+ * volume weighted (sum of Q/aj): out = {mass, x-, y-, z-momentum, total
+ * energy, kinetic energy}.  No counterpart in the reference: it serves the
+ * conservation tests and the kinetic-energy dissipation rate of the
+ * Taylor-Green runs.  Deterministic summation order. */
+int cles_integrals(cles_ctx *c, double out[6]);
+
+/* ---------------------------------------------------------------------------
+ * Batched serving.  No counterpart in the reference: several independent
+ * periodic solves (any size, Mach, Reynolds number; no LES model, no immersed
+ * body; RK2 or RK3) are advanced together by batched stage and seam kernels
+ * whose workgroups cover every member's tiles, captured once as a hipGraph
+ * per step.  Members are ordinary contexts with their state uploaded; each
+ * member's result equals what cles_advance gives for it alone.
+ * ------------------------------------------------------------------------ */
+typedef struct cles_batch cles_batch;
+/* Device bytes cles_create allocates for p (the admission footprint). */
+size_t cles_footprint(const cles_params *p);
+/* A batch that admits members while their footprints fit hbm_budget bytes. */
+cles_batch *cles_batch_create(size_t hbm_budget, int *err);
+void cles_batch_destroy(cles_batch *b);
+/* Admit / release a member; CLES_ENOMEM when c does not fit the budget. */
+int cles_batch_add(cles_batch *b, cles_ctx *c);
+int cles_batch_remove(cles_batch *b, cles_ctx *c);
+int cles_batch_size(const cles_batch *b);
+size_t cles_batch_used(const cles_batch *b);
+/* Stage-kernel tile shape of the batched launches: 0 32x4x2, 1 16x4x4,
+ * 2 16x8x2, 3 64x2x2 (chosen by the autotuner of the serving loop). */
+int cles_batch_set_tile(cles_batch *b, int tile);
+/* Advance every member by nsteps time steps (one graph replay per step) and
+ * wait for them. */
+int cles_batch_advance(cles_batch *b, int nsteps);
+/* The same, returning once the steps are queued; the host may then set up or
+ * download other contexts.  Members must not be added, removed or read
+ * before cles_batch_wait. */
+int cles_batch_launch(cles_batch *b, int nsteps);
+int cles_batch_wait(cles_batch *b);
+/* cles_batch_advance(b, nrep) timed with HIP events: mean ms per step. */
+int cles_batch_time_step(cles_batch *b, int nrep, double *ms);
+const char *cles_batch_last_error(const cles_batch *b);
+/* Stage-kernel tile shape of a single context (same codes as above). */
+int cles_set_tile(cles_ctx *c, int tile);
+
 /* Device bytes held by the context. */
 size_t cles_device_bytes(const cles_ctx *c);
 /* Last error message of the context (static storage when c is NULL). */

@@ -1,0 +1,114 @@
+"""Batched serving on the GPU: a member's result is the same alone and in a
+mix of other solves, whatever the tile shape, the batch composition or the
+step at which it joined; admission respects the HBM budget.  Results are
+compared bit for bit with the single-context path (cles_advance)."""
+import numpy as np
+import pytest
+
+import chimera_comp_les_amd as cl
+from chimera_comp_les_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def request(kind, N, Ma, Re=None, prec=cl.PREC_FP32BULK):
+    if kind == "vortex":
+        cfg, coor, Q = cases.vortex_case(N=N, mach=Ma)
+    else:
+        cfg, coor, Q = cases.tgv_case(N=N, Re=Re, Ma=Ma, les=0)
+    return cfg, coor, Q, prec
+
+
+def solo(req, steps):
+    cfg, coor, Q, prec = req
+    s = cl.Solver(cfg, coor, precision=prec)
+    s.set_state(Q)
+    s.advance(steps)
+    out = s.state()
+    s.close()
+    return out
+
+
+def member(req):
+    cfg, coor, Q, prec = req
+    s = cl.Solver(cfg, coor, precision=prec)
+    s.set_state(Q)
+    return s
+
+
+MIX = [("vortex", 24, 0.3, None, cl.PREC_FP32BULK), ("tgv", 20, 0.1, 400.0, cl.PREC_FP32BULK),
+       ("vortex", 16, 0.5, None, cl.PREC_FP64), ("tgv", 28, 0.4, 3000.0, cl.PREC_FP64),
+       ("tgv", 16, 0.2, 150.0, cl.PREC_FP32BULK)]
+
+
+def test_footprint_is_the_allocation():
+    for kind, N, Ma, Re, prec in MIX[:2]:
+        cfg, coor, Q, _ = request(kind, N, Ma, Re)
+        s = cl.Solver(cfg, coor, precision=prec)
+        assert s.device_bytes() == cl.footprint(cfg, prec)
+        s.close()
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+def test_mix_equals_solo(tile):
+    reqs = [request(*m) for m in MIX]
+    ref = [solo(r, 4) for r in reqs]
+    b = cl.Batch(1 << 40)
+    b.set_tile(tile)
+    ms = [member(r) for r in reqs]
+    for s in ms:
+        assert b.add(s)
+    b.advance(4)
+    for s, (Qr, Pr) in zip(ms, ref):
+        Qg, Pg = s.state()
+        assert np.array_equal(Qg, Qr) and np.array_equal(Pg, Pr)
+        assert s.ti == 4
+    b.close()
+    for s in ms:
+        s.close()
+
+
+def test_join_midway_and_leave():
+    reqs = [request(*m) for m in MIX[:3]]
+    b = cl.Batch(1 << 40)
+    m0, m1 = member(reqs[0]), member(reqs[1])
+    assert b.add(m0) and b.add(m1)
+    b.advance(2)
+    m2 = member(reqs[2])          # arrives while the others are in flight
+    assert b.add(m2)
+    b.advance(3)
+    b.remove(m0)                  # m0 finished after 5 steps
+    b.advance(1)
+    Q0 = m0.state()[0]
+    assert np.array_equal(Q0, solo(reqs[0], 5)[0])
+    assert np.array_equal(m1.state()[0], solo(reqs[1], 6)[0])
+    assert np.array_equal(m2.state()[0], solo(reqs[2], 4)[0])
+    b.close()
+    for s in (m0, m1, m2):
+        s.close()
+
+
+def test_admission_by_footprint():
+    reqs = [request(*m) for m in MIX[:2]]
+    need = [cl.footprint(r[0], r[3]) for r in reqs]
+    b = cl.Batch(need[0] + need[1] // 2)
+    m0, m1 = member(reqs[0]), member(reqs[1])
+    assert b.add(m0)
+    assert not b.add(m1)          # does not fit beside m0
+    b.remove(m0)
+    assert b.add(m1)              # fits once m0 has left
+    assert b.used() == need[1]
+    b.close()
+    m0.close()
+    m1.close()
+
+
+def test_rejects_unsupported_members():
+    cfg, coor, Q = cases.tgv_case(N=16, les=2)
+    s = cl.Solver(cfg, coor)
+    s.set_state(Q)
+    b = cl.Batch(1 << 40)
+    with pytest.raises(cl.CLESError):
+        b.add(s)
+    b.close()
+    s.close()
