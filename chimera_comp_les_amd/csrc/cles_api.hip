@@ -240,6 +240,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
   const long long N = (long long)g.mx * g.my * g.mz;
   g.np = (N + 63) / 64 * 64;
   c->ucat_src = -1;
+  c->tile = 1;  // 16x4x4: fewest faces and window nodes per interior node of the four shapes
   c->ti = p->tistart;
   sigma_spec(c);
 #define TRY(x)                          \

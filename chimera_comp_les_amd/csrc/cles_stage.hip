@@ -29,6 +29,11 @@ namespace cles {
 namespace {
 
 constexpr int NT = 256;    // threads per workgroup, one interior node each
+// occupancy target of the stage kernels (waves per SIMD the register
+// allocation must allow)
+#ifndef CLES_STAGE_WAVES
+#define CLES_STAGE_WAVES 2
+#endif
 constexpr int NSLOT = 64;  // spread of the inf-norm atomics
 
 // Tile of interior nodes owned by one workgroup, with the face counts of its
@@ -41,7 +46,7 @@ struct Tile {
   static constexpr int NF = NFI + NFJ + NFK;
   static constexpr int WX = X + 2, WY = Y + 2, WZ = Z + 2, NW = WX * WY * WZ;
 };
-// tile shapes the launcher can pick (index = cles_ctx::tile)
+// tile shapes the launcher can pick (index = cles_ctx::tile, default 1)
 typedef Tile<32, 4, 2> TileA;
 typedef Tile<16, 4, 4> TileB;
 typedef Tile<16, 8, 2> TileC;
@@ -640,9 +645,10 @@ template <class TL> __device__ __forceinline__ int fk_of(int lx, int ly, int lz)
 // One workgroup's tile of one grid; b only spreads the norm atomics.
 template <class TL, typename T, bool LES, bool VISC, int MODE>
 __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) {
-  constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, NF = TL::NF, WX = TL::WX, WY = TL::WY, NW = TL::NW;
+  constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, WX = TL::WX, WY = TL::WY, NW = TL::NW;
   constexpr int FC = VISC ? (LES ? 7 : 5) : 5;
-  __shared__ double Fs[FC][NF];
+  constexpr int NFD = TL::NFI > TL::NFJ ? (TL::NFI > TL::NFK ? TL::NFI : TL::NFK) : (TL::NFJ > TL::NFK ? TL::NFJ : TL::NFK);
+  __shared__ double Fs[FC][NFD];
   __shared__ T Ws[VISC ? 4 * NW : 1];
   const Grid &g = a.g;
   if (tile >= a.ntx * a.nty * a.ntz) return;
@@ -651,46 +657,21 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   const int t = threadIdx.x;
   const long long np = g.np;
 
-  // ---- phase A: convective face fluxes -> LDS
-  for (int f = t; f < NF; f += NT) {
-    int d, lx, ly, lz;
-    decode_face<TL>(f, d, lx, ly, lz);
-    int gi = i0 + lx - (d == 0), gj = j0 + ly - (d == 1), gk = k0 + lz - (d == 2);
-    bool ok = d == 0 ? (gi <= g.mx - 2 && gj <= g.my - 2 && gk <= g.mz - 2)
-                     : d == 1 ? (gj <= g.my - 2 && gi <= g.mx - 2 && gk <= g.mz - 2)
-                              : (gk <= g.mz - 2 && gi <= g.mx - 2 && gj <= g.my - 2);
-    if (!ok) continue;
-    double F[5];
-    if (LES) conv_face_les<T>(a, d, gk, gj, gi, F);
-    else conv_face_plain<T>(a, d, gk, gj, gi, F);
-#pragma unroll
-    for (int q = 0; q < 5; q++) Fs[q][f] = F[q];
-  }
-  __syncthreads();
-
-  // ---- phase B: convective residual of this thread's node
+  // Faces are processed one direction at a time (convective, then viscous),
+  // so the face buffer holds one direction's faces; the per-node sums run in
+  // the order i, j, k exactly as a single expression over all six faces would.
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
   const int gi = i0 + lx, gj = j0 + ly, gk = k0 + lz;
   const bool valid = gi <= g.mx - 2 && gj <= g.my - 2 && gk <= g.mz - 2;
   const long long n = valid ? nid(g, gk, gj, gi) : 0;
-  double conv[5];
-  {
-    const int a0 = fi_of<TL>(lx, ly, lz), a1 = fi_of<TL>(lx + 1, ly, lz);
-    const int b0 = fj_of<TL>(lx, ly, lz), b1 = fj_of<TL>(lx, ly + 1, lz);
-    const int c0 = fk_of<TL>(lx, ly, lz), c1 = fk_of<TL>(lx, ly, lz + 1);
-#pragma unroll
-    for (int q = 0; q < 5; q++)
-      conv[q] = double(Fs[q][a1] - Fs[q][a0] + Fs[q][b1] - Fs[q][b0] + Fs[q][c1] - Fs[q][c0]);
-    if (LES && a.ratio != 0.0 && valid) {
-      conv[3] = conv[3] - a.ratio / a.aj[n] * a.q[n] / a.dt;
-      conv[4] = conv[4] - a.ratio / a.aj[n] * a.q[n + 3 * np] / a.dt;
-    }
-  }
+  double conv[5] = {0., 0., 0., 0., 0.};
   double visc[5] = {0., 0., 0., 0., 0.};
+  // LES energy term (rhs.c:2153-2197): face-density weighted alpha sums
+  const bool les_e = LES && VISC && valid && gj > 2 && gj < g.my - 2 && !(a.has_ib && !(a.nv[n] < 0.5));
+  T E0 = T(0), E1 = T(0), E2 = T(0);
+  Win win{i0 - 1, j0 - 1, k0 - 1};
   if (VISC) {
-    __syncthreads();  // everyone is done reading the convective faces
-    // ---- phase C1: stage u, v, w, p/rho of the tile and a one-node halo
-    Win win{i0 - 1, j0 - 1, k0 - 1};
+    // stage u, v, w, p/rho of the tile and a one-node halo
     for (int w = t; w < NW; w += NT) {
       int wx = w % WX, wy = (w / WX) % WY, wz = w / (WX * WY);
       int ii = win.i0 + wx, jj = win.j0 + wy, kk = win.k0 + wz;
@@ -702,14 +683,35 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       Ws[2 * NW + w] = T(a.q[m + 3 * np] / r);
       Ws[3 * NW + w] = T(a.p[m] / r - a.t_ref);  // t_ref = 0 in FP64: only gradients of p/rho are used
     }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {  // unrolled: d is a constant in the face routines
+    const int nfd = d == 0 ? TL::NFI : d == 1 ? TL::NFJ : TL::NFK;
+    const int ex = TX + (d == 0), ey = TY + (d == 1);  // face-grid extents of direction d
+    // faces of this node: lo at the node, hi one step along d
+    const int lo = (lz * ey + ly) * ex + lx;
+    const int hi = lo + (d == 0 ? 1 : d == 1 ? ex : ex * ey);
+    // ---- convective fluxes of direction d -> LDS
+    for (int f = t; f < nfd; f += NT) {
+      const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
+      const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
+      if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
+      double F[5];
+      if (LES) conv_face_les<T>(a, d, fk, fj, fi, F);
+      else conv_face_plain<T>(a, d, fk, fj, fi, F);
+#pragma unroll
+      for (int q = 0; q < 5; q++) Fs[q][f] = F[q];
+    }
     __syncthreads();
-    // ---- phase C2: viscous face fluxes -> LDS
-    for (int f = t; f < NF; f += NT) {
-      int d, fx, fy, fz;
-      decode_face<TL>(f, d, fx, fy, fz);
-      int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
-      bool ok = fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2;
-      if (!ok) continue;
+#pragma unroll
+    for (int q = 0; q < 5; q++) conv[q] = d == 0 ? Fs[q][hi] - Fs[q][lo] : conv[q] + Fs[q][hi] - Fs[q][lo];
+    __syncthreads();
+    if (!VISC) continue;
+    // ---- viscous fluxes of direction d -> LDS
+    for (int f = t; f < nfd; f += NT) {
+      const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
+      const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
+      if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
       T G[4], al[3] = {T(0), T(0), T(0)};
       visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, G, al);
 #pragma unroll
@@ -720,25 +722,30 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       }
     }
     __syncthreads();
-    // ---- phase D1: viscous residual (+ LES energy term, rhs.c:2153-2197)
-    const int a0 = fi_of<TL>(lx, ly, lz), a1 = fi_of<TL>(lx + 1, ly, lz);
-    const int b0 = fj_of<TL>(lx, ly, lz), b1 = fj_of<TL>(lx, ly + 1, lz);
-    const int c0 = fk_of<TL>(lx, ly, lz), c1 = fk_of<TL>(lx, ly, lz + 1);
 #pragma unroll
     for (int q = 0; q < 4; q++)
-      visc[q + 1] = double(Fs[q][a1] - Fs[q][a0] + Fs[q][b1] - Fs[q][b0] + Fs[q][c1] - Fs[q][c0]);
-    if (LES && valid && gj > 2 && gj < g.my - 2 && !(a.has_ib && !(a.nv[n] < 0.5))) {
+      visc[q + 1] = d == 0 ? Fs[q][hi] - Fs[q][lo] : visc[q + 1] + Fs[q][hi] - Fs[q][lo];
+    if (les_e) {
+      // face densities (r_n + r_n-1)/2, (r_n+1 + r_n)/2 along d
+      const long long st = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
       const double *r = a.q;
-      const long long sy = g.sy, sz = g.sz;
-      T rL = T((r[n] + r[n - 1]) * 0.5), rR = T((r[n + 1] + r[n]) * 0.5);
-      T rD = T((r[n] + r[n - sy]) * 0.5), rU = T((r[n + sy] + r[n]) * 0.5);
-      T rB = T((r[n] + r[n - sz]) * 0.5), rF = T((r[n + sz] + r[n]) * 0.5);
-      T u0 = T(a.q[n + np] / r[n]), u1 = T(a.q[n + 2 * np] / r[n]), u2 = T(a.q[n + 3 * np] / r[n]);
-      T e = u0 * (rR * Fs[4][a1] - rL * Fs[4][a0] + rU * Fs[4][b1] - rD * Fs[4][b0] + rF * Fs[4][c1] - rB * Fs[4][c0]) +
-            u1 * (rR * Fs[5][a1] - rL * Fs[5][a0] + rU * Fs[5][b1] - rD * Fs[5][b0] + rF * Fs[5][c1] - rB * Fs[5][c0]) +
-            u2 * (rR * Fs[6][a1] - rL * Fs[6][a0] + rU * Fs[6][b1] - rD * Fs[6][b0] + rF * Fs[6][c1] - rB * Fs[6][c0]);
-      visc[4] = visc[4] + double(e);
+      const T rl = T((r[n] + r[n - st]) * 0.5), rh = T((r[n + st] + r[n]) * 0.5);
+      const T p0 = rh * T(Fs[4][hi]), p1 = rh * T(Fs[5][hi]), p2 = rh * T(Fs[6][hi]);
+      E0 = d == 0 ? p0 - rl * T(Fs[4][lo]) : E0 + p0 - rl * T(Fs[4][lo]);
+      E1 = d == 0 ? p1 - rl * T(Fs[5][lo]) : E1 + p1 - rl * T(Fs[5][lo]);
+      E2 = d == 0 ? p2 - rl * T(Fs[6][lo]) : E2 + p2 - rl * T(Fs[6][lo]);
     }
+    __syncthreads();
+  }
+  if (LES && a.ratio != 0.0 && valid) {
+    conv[3] = conv[3] - a.ratio / a.aj[n] * a.q[n] / a.dt;
+    conv[4] = conv[4] - a.ratio / a.aj[n] * a.q[n + 3 * np] / a.dt;
+  }
+  if (les_e) {
+    const double rn = a.q[n];
+    const T u0 = T(a.q[n + np] / rn), u1 = T(a.q[n + 2 * np] / rn), u2 = T(a.q[n + 3 * np] / rn);
+    const T e = u0 * E0 + u1 * E1 + u2 * E2;
+    visc[4] = visc[4] + double(e);
   }
   if (!valid) return;
 
@@ -790,7 +797,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
 }
 
 template <class TL, typename T, bool LES, bool VISC, int MODE>
-__global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CLES_STAGE_WAVES))) void stage_kernel(StageArgs a) {
   // XCD-aware tile numbering: blockIdx % 8 selects the XCD, each XCD walks a
   // contiguous range of tiles and keeps their shared halos in its own L2.
   const int b = blockIdx.x;
@@ -801,7 +808,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
 // independent grids; tstart[r] is the first workgroup of grid r (prefix sum
 // of the tile counts, tstart[nreq] = total).
 template <class TL, typename T, bool LES, bool VISC, int MODE>
-__global__ __launch_bounds__(NT) void stage_batch_kernel(const StageArgs *__restrict__ tab, const int *__restrict__ tstart,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CLES_STAGE_WAVES))) void stage_batch_kernel(const StageArgs *__restrict__ tab, const int *__restrict__ tstart,
                                                          int nreq) {
   const int b = blockIdx.x;
   int lo = 0, hi = nreq;  // largest r with tstart[r] <= b
