@@ -333,7 +333,7 @@ class Solver:
         return out
 
     def set_tile(self, tile):
-        """Stage-kernel tile shape: 0 32x4x2, 1 16x4x4, 2 16x8x2, 3 64x2x2."""
+        """Stage-kernel tile shape: 0 32x4x2, 1 16x4x4, 2 16x8x2, 3 64x2x2, 4 6x6x6, 5 8x5x5."""
         self._check(self.L.cles_set_tile(self.h, int(tile)))
 
 

@@ -240,7 +240,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
   const long long N = (long long)g.mx * g.my * g.mz;
   g.np = (N + 63) / 64 * 64;
   c->ucat_src = -1;
-  c->tile = 1;  // 16x4x4: fewest faces and window nodes per interior node of the four shapes
+  c->tile = 4;  // 6x6x6: each direction's faces fit one pass of the workgroup
   c->ti = p->tistart;
   sigma_spec(c);
 #define TRY(x)                          \
@@ -591,7 +591,7 @@ int cles_time_les(cles_ctx *c, int nrep, double *ms) {
 size_t cles_device_bytes(const cles_ctx *c) { return c ? c->bytes : 0; }
 
 int cles_set_tile(cles_ctx *c, int tile) {
-  if (!c || tile < 0 || tile > 3) return CLES_EINVAL;
+  if (!c || tile < 0 || tile > 5) return CLES_EINVAL;
   c->tile = tile;
   return CLES_OK;
 }

@@ -226,7 +226,7 @@ int cles_batch_size(const cles_batch *b) { return b ? (int)b->members.size() : C
 size_t cles_batch_used(const cles_batch *b) { return b ? b->used : 0; }
 
 int cles_batch_set_tile(cles_batch *b, int tile) {
-  if (!b || tile < 0 || tile > 3) return CLES_EINVAL;
+  if (!b || tile < 0 || tile > 5) return CLES_EINVAL;
   if (tile != b->tile) {
     b->tile = tile;
     b->dirty = true;
@@ -269,16 +269,17 @@ int cles_batch_time_step(cles_batch *b, int nrep, double *ms) {
     int r = rebuild(b);
     if (r) return r;
   }
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return batch_error(b, CLES_EDEVICE, "event");
-  hipEventRecord(e0, b->stream);
-  for (int q = 0; q < nrep; q++) hipGraphLaunch(b->exec, b->stream);
-  hipEventRecord(e1, b->stream);
-  hipError_t e = hipEventSynchronize(e1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+  for (int q = 0; q < nrep && e == hipSuccess; q++) e = hipGraphLaunch(b->exec, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
   float t = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (e != hipSuccess) return batch_error(b, CLES_EDEVICE, "batch timing");
   *ms = (double)t / nrep;
   for (cles_ctx *c : b->members) c->ti += nrep;

@@ -9,14 +9,16 @@
 // in registers, so the only HBM traffic is the stage input, the metrics and
 // the stage output.
 //
-// Work decomposition: a workgroup of 256 threads owns a 32x4x2 tile of
-// interior nodes.  Phase A computes the convective fluxes of the
-// 33*4*2 + 32*5*2 + 32*4*3 = 968 faces touching the tile into LDS, phase B
-// forms the convective residual per node.  Phase C stages the primitive
-// variables (u, v, w, p/rho) of the tile plus a one-node halo in LDS and
-// computes the viscous face fluxes into the same LDS face buffer, phase D
-// forms the viscous residual and the stage update.  Faces are computed once
-// per workgroup (1.26 faces per node-face on average) instead of twice per
+// Work decomposition: a workgroup of 256 threads owns a tile of interior
+// nodes (one node per thread; 6x6x6 by default, five other shapes).  The
+// primitive variables (u, v, w, p/rho) of the tile plus a one-node halo are
+// staged in LDS first.  Then, one direction at a time: the convective fluxes
+// of the direction's faces (6x6x6: 7*6*6 = 252, one pass of the workgroup)
+// go to an LDS face buffer and each node adds F(hi) - F(lo); the viscous
+// fluxes of the same faces follow through the same buffer.  For the FP64
+// central flux of the LES form the direction's node data (state, p and the
+// contravariant velocity) are staged in LDS as well, so each node's division
+// is done once.  Faces are computed once per workgroup instead of twice per
 // node.  Tiles are numbered so that each XCD (blockIdx % 8) receives a
 // contiguous slab of tiles and keeps its neighbours in its own L2.
 #include <cstring>
@@ -40,8 +42,8 @@ constexpr int NSLOT = 64;  // spread of the inf-norm atomics
 // convective/viscous face buffer and its primitive window (one-node halo).
 template <int X, int Y, int Z>
 struct Tile {
-  static_assert(X * Y * Z == NT, "one node per thread");
-  static constexpr int TX = X, TY = Y, TZ = Z;
+  static_assert(X * Y * Z <= NT, "at most one node per thread");
+  static constexpr int TX = X, TY = Y, TZ = Z, NN = X * Y * Z;
   static constexpr int NFI = (X + 1) * Y * Z, NFJ = X * (Y + 1) * Z, NFK = X * Y * (Z + 1);
   static constexpr int NF = NFI + NFJ + NFK;
   static constexpr int WX = X + 2, WY = Y + 2, WZ = Z + 2, NW = WX * WY * WZ;
@@ -51,6 +53,10 @@ typedef Tile<32, 4, 2> TileA;
 typedef Tile<16, 4, 4> TileB;
 typedef Tile<16, 8, 2> TileC;
 typedef Tile<64, 2, 2> TileD;
+// shapes whose face count per direction fits one pass of the workgroup
+// (6x6x6: 252 faces; 8x5x5: at most 240), at 216 and 200 nodes per workgroup
+typedef Tile<6, 6, 6> TileE;
+typedef Tile<8, 5, 5> TileF;
 
 struct StageArgs {
   Grid g;
@@ -83,6 +89,28 @@ __device__ __forceinline__ void load6(const StageArgs &a, long long n, T s[6]) {
   s[4] = T(a.q[n + 4 * np]);
   s[5] = T(a.p[n]);
 }
+// Node data of the current face direction staged in LDS by the workgroup:
+// S[c * ns + o], c = 0..6 for rho, rhoU, rhoV, rhoW, rhoE, p and the
+// contravariant velocity of the direction; o0 is the slot of the face's lower
+// node, st the slot step along the direction.  S == nullptr: read global memory.
+struct Sv {
+  const double *S;
+  int ns, o0, st;
+  bool on;  // staged (S valid)
+};
+
+// the six state values of node n, which sits dq nodes from the face's lower node
+template <typename T>
+__device__ __forceinline__ void load6v(const StageArgs &a, const Sv &sv, long long n, int dq, T s[6]) {
+  if (sv.on) {
+    const int o = sv.o0 + dq * sv.st;
+#pragma unroll
+    for (int c = 0; c < 6; c++) s[c] = T(sv.S[c * sv.ns + o]);
+  } else {
+    load6(a, n, s);
+  }
+}
+
 __device__ __forceinline__ double nvat(const StageArgs &a, long long n) { return a.has_ib ? a.nv[n] : 0.0; }
 __device__ __forceinline__ const double *metric(const StageArgs &a, int d) {
   return d == 0 ? a.csi : d == 1 ? a.eta : a.zet;
@@ -95,13 +123,13 @@ __device__ __forceinline__ double sigma_at(const StageArgs &a, int d, int k, int
 }
 
 template <typename T>
-__device__ __forceinline__ void recon_muscl(const StageArgs &a, long long nm, long long n0, long long n1,
+__device__ __forceinline__ void recon_muscl(const StageArgs &a, const Sv &sv, long long nm, long long n0, long long n1,
                                             long long n2, T *L, T *R) {
   T s0[6], s1[6], sm[6], sp[6];
-  load6(a, n0, s0);
-  load6(a, n1, s1);
-  if (nvat(a, nm) > 2.9) load6(a, n1, sm); else load6(a, nm, sm);
-  if (nvat(a, n2) > 2.9) load6(a, n0, sp); else load6(a, n2, sp);
+  load6v(a, sv, n0, 0, s0);
+  load6v(a, sv, n1, 1, s1);
+  if (nvat(a, nm) > 2.9) load6v(a, sv, n1, 1, sm); else load6v(a, sv, nm, -1, sm);
+  if (nvat(a, n2) > 2.9) load6v(a, sv, n0, 0, sp); else load6v(a, sv, n2, 2, sp);
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     L[q] = muscl_ql(sm[q], s0[q], s1[q]);
@@ -110,13 +138,13 @@ __device__ __forceinline__ void recon_muscl(const StageArgs &a, long long nm, lo
 }
 
 template <typename T>
-__device__ __forceinline__ void recon_weno(const StageArgs &a, long long nm, long long n0, long long n1,
+__device__ __forceinline__ void recon_weno(const StageArgs &a, const Sv &sv, long long nm, long long n0, long long n1,
                                            long long n2, bool eno_left, int eno_mask, T *L, T *R) {
   T sm[6], s0[6], s1[6], s2[6];
-  load6(a, nm, sm);
-  load6(a, n0, s0);
-  load6(a, n1, s1);
-  load6(a, n2, s2);
+  load6v(a, sv, nm, -1, sm);
+  load6v(a, sv, n0, 0, s0);
+  load6v(a, sv, n1, 1, s1);
+  load6v(a, sv, n2, 2, s2);
   const int order = a.eno;
 #pragma unroll
   for (int q = 0; q < 6; q++) {
@@ -191,17 +219,17 @@ __device__ __forceinline__ void load6d(const StageArgs &a, long long n, double s
 // FP32-bulk reconstruction: WENO / ENO / MUSCL are equivariant under a common
 // shift of their three inputs, so they run in float on the deviations from
 // the node n0 value and the shift is added back in double.
-__device__ void recon_split(const StageArgs &a, bool muscl, long long nm, long long n0, long long n1, long long n2,
-                            bool eno_left, int eno_mask, double L[6], double R[6]) {
+__device__ void recon_split(const StageArgs &a, const Sv &sv, bool muscl, long long nm, long long n0, long long n1,
+                            long long n2, bool eno_left, int eno_mask, double L[6], double R[6]) {
   double sm[6], s0[6], s1[6], s2[6];
-  load6d(a, n0, s0);
-  load6d(a, n1, s1);
+  load6v(a, sv, n0, 0, s0);
+  load6v(a, sv, n1, 1, s1);
   if (muscl) {
-    if (nvat(a, nm) > 2.9) load6d(a, n1, sm); else load6d(a, nm, sm);
-    if (nvat(a, n2) > 2.9) load6d(a, n0, s2); else load6d(a, n2, s2);
+    if (nvat(a, nm) > 2.9) load6v(a, sv, n1, 1, sm); else load6v(a, sv, nm, -1, sm);
+    if (nvat(a, n2) > 2.9) load6v(a, sv, n0, 0, s2); else load6v(a, sv, n2, 2, s2);
   } else {
-    load6d(a, nm, sm);
-    load6d(a, n2, s2);
+    load6v(a, sv, nm, -1, sm);
+    load6v(a, sv, n2, 2, s2);
   }
   const bool solid2 = nvat(a, n2) > 2.9;
 #pragma unroll
@@ -222,7 +250,7 @@ __device__ void recon_split(const StageArgs &a, bool muscl, long long nm, long l
 
 // formConvection face (rhs.c:2646-2814, 2817-2971, 2973-3121)
 template <typename T>
-__device__ void conv_face_plain(const StageArgs &a, int d, int k, int j, int i, double F[5]) {
+__device__ void conv_face_plain(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]) {
   const Grid &g = a.g;
   const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
   const int id = d == 0 ? i : d == 1 ? j : k;
@@ -233,14 +261,14 @@ __device__ void conv_face_plain(const StageArgs &a, int d, int k, int j, int i, 
   typedef typename std::conditional<F32, double, T>::type S;  // face-state type
   S L[6], R[6];
   if (d == 1 && j == 0 && iso && a.bc[2] == 1) {
-    load6(a, n0, L);
-    load6(a, n0, R);
+    load6v(a, sv, n0, 0, L);
+    load6v(a, sv, n0, 0, R);
   } else if (d == 1 && j == g.my - 2 && iso && a.bc[3] == 1) {
-    load6(a, n1, L);
-    load6(a, n1, R);
+    load6v(a, sv, n1, 1, L);
+    load6v(a, sv, n1, 1, R);
   } else if (a.fluxorder == 1 || id == 0 || id == m - 2 || (nvat(a, n0) + nvat(a, n1)) > 2.9) {
-    load6(a, n0, L);
-    load6(a, n1, R);
+    load6v(a, sv, n0, 0, L);
+    load6v(a, sv, n1, 1, R);
   } else {
     const long long nm = n0 - s, n2 = n0 + 2 * s;
     const bool muscl = a.fluxorder == 2 || (d != 0 && j < 30 && k < 7);
@@ -252,10 +280,10 @@ __device__ void conv_face_plain(const StageArgs &a, int d, int k, int j, int i, 
       if (d == 1 && !eno_left && j == 1 && iso && a.bc[2] == 1) mask = (1 << 0) | (1 << 4) | (1 << 5);
     }
     if constexpr (F32) {
-      recon_split(a, muscl, nm, n0, n1, n2, eno_left, mask, L, R);
+      recon_split(a, sv, muscl, nm, n0, n1, n2, eno_left, mask, L, R);
     } else {
-      if (muscl) recon_muscl(a, nm, n0, n1, n2, L, R);
-      else recon_weno(a, nm, n0, n1, n2, eno_left, mask, L, R);
+      if (muscl) recon_muscl(a, sv, nm, n0, n1, n2, L, R);
+      else recon_weno(a, sv, nm, n0, n1, n2, eno_left, mask, L, R);
     }
   }
   S h[3];
@@ -275,13 +303,13 @@ __device__ void conv_face_plain(const StageArgs &a, int d, int k, int j, int i, 
 }
 
 template <typename T>
-__device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, double F[5]);
+__device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]);
 
 // FP32-bulk four-point central flux (sigma = 1).  fp_2(c, U) = c * fp4(U) for
 // a constant c, so rho, rhoE + p and the pressure are split into reference
 // parts, whose fluxes rho_ref fp4(U), h_ref fp4(U) and p_ref h use the
 // contravariant velocity in double, and float deviations for the rest.
-__device__ bool conv_face_les_split(const StageArgs &a, int d, int k, int j, int i, double F[5]) {
+__device__ bool conv_face_les_split(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]) {
   const Grid &g = a.g;
   const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
   const int id = d == 0 ? i : d == 1 ? j : k;
@@ -302,9 +330,17 @@ __device__ bool conv_face_les_split(const StageArgs &a, int d, int k, int j, int
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const long long n = nn[q];
-    const double rd = a.q[n], ru = a.q[n + np], rv = a.q[n + 2 * np], rw = a.q[n + 3 * np];
-    const double ed = a.q[n + 4 * np], pd = a.p[n];
-    Ud[q] = (mt[n] * ru + mt[n + np] * rv + mt[n + 2 * np] * rw) * inv_refined(rd);
+    double rd, ru, rv, rw, ed, pd;
+    if (sv.on) {
+      const int o = sv.o0 + (q - 1) * sv.st;
+      rd = sv.S[o]; ru = sv.S[sv.ns + o]; rv = sv.S[2 * sv.ns + o]; rw = sv.S[3 * sv.ns + o];
+      ed = sv.S[4 * sv.ns + o]; pd = sv.S[5 * sv.ns + o];
+      Ud[q] = sv.S[6 * sv.ns + o];
+    } else {
+      rd = a.q[n]; ru = a.q[n + np]; rv = a.q[n + 2 * np]; rw = a.q[n + 3 * np];
+      ed = a.q[n + 4 * np]; pd = a.p[n];
+      Ud[q] = (mt[n] * ru + mt[n + np] * rv + mt[n + 2 * np] * rw) * inv_refined(rd);
+    }
     U[q] = (float)Ud[q];
     r[q] = (float)(rd - a.rho_ref);
     u[q] = (float)ru;
@@ -342,9 +378,10 @@ __device__ bool conv_face_les_split(const StageArgs &a, int d, int k, int j, int
 // it does not cover (boundaries, immersed nodes, sigma < 1), this function in
 // double.
 template <typename T>
-__device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, double F[5]) {
+__device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]) {
   if constexpr (sizeof(T) == 4) {
-    if (!conv_face_les_split(a, d, k, j, i, F)) conv_face_les<double>(a, d, k, j, i, F);
+    // the double fallback reads global memory: the staged velocity is the float form's
+    if (!conv_face_les_split(a, sv, d, k, j, i, F)) conv_face_les<double>(a, Sv{nullptr, 0, 0, 0, false}, d, k, j, i, F);
     return;
   }
   const Grid &g = a.g;
@@ -369,14 +406,26 @@ __device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, do
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const long long n = nn[q];
-        r[q] = T(a.q[n]);
-        u[q] = T(a.q[n + np]);
-        v[q] = T(a.q[n + 2 * np]);
-        w[q] = T(a.q[n + 3 * np]);
-        T e = T(a.q[n + 4 * np]);
-        pp[q] = T(a.p[n]);
+        T e;
+        if (sv.on) {
+          const int o = sv.o0 + (q - 1) * sv.st;
+          r[q] = T(sv.S[o]);
+          u[q] = T(sv.S[sv.ns + o]);
+          v[q] = T(sv.S[2 * sv.ns + o]);
+          w[q] = T(sv.S[3 * sv.ns + o]);
+          e = T(sv.S[4 * sv.ns + o]);
+          pp[q] = T(sv.S[5 * sv.ns + o]);
+          U[q] = T(sv.S[6 * sv.ns + o]);
+        } else {
+          r[q] = T(a.q[n]);
+          u[q] = T(a.q[n + np]);
+          v[q] = T(a.q[n + 2 * np]);
+          w[q] = T(a.q[n + 3 * np]);
+          e = T(a.q[n + 4 * np]);
+          pp[q] = T(a.p[n]);
+          U[q] = T(1.) / r[q] * (T(mt[n]) * u[q] + T(mt[n + np]) * v[q] + T(mt[n + 2 * np]) * w[q]);
+        }
         ep[q] = d == 2 ? e + pp[q] : pp[q] + e;
-        U[q] = T(1.) / r[q] * (T(mt[n]) * u[q] + T(mt[n + np]) * v[q] + T(mt[n + 2 * np]) * w[q]);
       }
       if (a.central == 2) {
         Fc[0] = fp_2(r, U);
@@ -395,8 +444,8 @@ __device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, do
   }
   if (!four) {
     T s0[6], s1[6];
-    load6(a, n0, s0);
-    load6(a, n1, s1);
+    load6v(a, sv, n0, 0, s0);
+    load6v(a, sv, n1, 1, s1);
     T rc = T(0.5) * (s0[0] + s1[0]);
     T uc = T(0.5) * (s0[1] + s1[1]);
     T vc = T(0.5) * (s0[2] + s1[2]);
@@ -423,15 +472,16 @@ __device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, do
   const long long n2 = nidw(g, k + 2 * (d == 2), j + 2 * (d == 1), i + 2 * (d == 0));
   const double nvpair = d == 2 ? nvat(a, n0) + nvat(a, nid(g, k + 1, j, 1)) : nvat(a, n0) + nvat(a, n1);
   T L[6], R[6];
+  const Sv gv{nullptr, 0, 0, 0, false};  // wrapped neighbours: read global memory
   if (a.fluxorder == 1 || ((id == 0 || id == m - 2) && a.bc[2 * d] != 7) || nvpair > 2.9) {
     load6(a, n0, L);
     load6(a, n1, R);
   } else if (a.fluxorder == 2) {
-    recon_muscl(a, nm, n0, n1, n2, L, R);
+    recon_muscl(a, gv, nm, n0, n1, n2, L, R);
   } else {
     bool eno_left = nvat(a, nm) > 2.9;
     if (d == 0) eno_left = eno_left && (i + 2) < g.mx;
-    recon_weno(a, nm, n0, n1, n2, eno_left, 0, L, R);
+    recon_weno(a, gv, nm, n0, n1, n2, eno_left, 0, L, R);
   }
   T Fw[5];
   rusanov(L, R, h, ucon_of(L, h), ucon_of(R, h), Fw);
@@ -440,7 +490,7 @@ __device__ void conv_face_les(const StageArgs &a, int d, int k, int j, int i, do
 }
 
 // instantiate the double version used by the float fallback
-template __device__ void conv_face_les<double>(const StageArgs &, int, int, int, int, double *);
+template __device__ void conv_face_les<double>(const StageArgs &, const Sv &, int, int, int, int, double *);
 
 // ------------------------------ viscous ----------------------------------
 struct Win {  // LDS window of primitives with a one-node halo
@@ -648,8 +698,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, WX = TL::WX, WY = TL::WY, NW = TL::NW;
   constexpr int FC = VISC ? (LES ? 7 : 5) : 5;
   constexpr int NFD = TL::NFI > TL::NFJ ? (TL::NFI > TL::NFK ? TL::NFI : TL::NFK) : (TL::NFJ > TL::NFK ? TL::NFJ : TL::NFK);
+  // staged node box of direction d: the tile widened by two nodes on both
+  // sides along d (the four-point stencils of its faces)
+  constexpr int NCH = LES ? 7 : 6;
+  constexpr int NS0 = (TX + 4) * TY * TZ, NS1 = TX * (TY + 4) * TZ, NS2 = TX * TY * (TZ + 4);
+  constexpr int NSM = NS0 > NS1 ? (NS0 > NS2 ? NS0 : NS2) : (NS1 > NS2 ? NS1 : NS2);
+  // staging pays where it saves the per-stencil-node division of the FP64
+  // central flux and two workgroups still fit in a CU's LDS; elsewhere the
+  // extra LDS costs more occupancy than the loads it saves
+  constexpr bool STG = LES && sizeof(T) == 8 &&
+                       8 * (FC * NFD + NCH * NSM) + (VISC ? (int)sizeof(T) * 4 * NW : 0) <= 80 * 1024;
   __shared__ double Fs[FC][NFD];
   __shared__ T Ws[VISC ? 4 * NW : 1];
+  __shared__ double Ss[STG ? NCH * NSM : 1];
   const Grid &g = a.g;
   if (tile >= a.ntx * a.nty * a.ntz) return;
   const int tix = tile % a.ntx, tiy = (tile / a.ntx) % a.nty, tiz = tile / (a.ntx * a.nty);
@@ -660,9 +721,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   // Faces are processed one direction at a time (convective, then viscous),
   // so the face buffer holds one direction's faces; the per-node sums run in
   // the order i, j, k exactly as a single expression over all six faces would.
-  const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
+  // threads beyond the tile's nodes take part in the face phases only (node
+  // coordinates clamped to 0 keep their LDS reads in range)
+  const bool tn = t < TL::NN;
+  const int lx = tn ? t % TX : 0, ly = tn ? (t / TX) % TY : 0, lz = tn ? t / (TX * TY) : 0;
   const int gi = i0 + lx, gj = j0 + ly, gk = k0 + lz;
-  const bool valid = gi <= g.mx - 2 && gj <= g.my - 2 && gk <= g.mz - 2;
+  const bool valid = tn && gi <= g.mx - 2 && gj <= g.my - 2 && gk <= g.mz - 2;
   const long long n = valid ? nid(g, gk, gj, gi) : 0;
   double conv[5] = {0., 0., 0., 0., 0.};
   double visc[5] = {0., 0., 0., 0., 0.};
@@ -691,14 +755,51 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     // faces of this node: lo at the node, hi one step along d
     const int lo = (lz * ey + ly) * ex + lx;
     const int hi = lo + (d == 0 ? 1 : d == 1 ? ex : ex * ey);
+    // ---- node data of direction d -> LDS: the state, p and (LES) the
+    // contravariant velocity, each computed once per node instead of once per
+    // stencil it belongs to
+    const int bx = TX + 4 * (d == 0), by = TY + 4 * (d == 1), bz = TZ + 4 * (d == 2);
+    const int ox = i0 - 2 * (d == 0), oy = j0 - 2 * (d == 1), oz = k0 - 2 * (d == 2);
+    constexpr int ns = NSM;
+    if (STG) {
+      const int nb = bx * by * bz;
+      const double *mt = metric(a, d);
+      for (int w = t; w < nb; w += NT) {
+        const int ii = ox + w % bx, jj = oy + (w / bx) % by, kk = oz + w / (bx * by);
+        if (ii < 0 || jj < 0 || kk < 0 || ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
+        const long long m = nid(g, kk, jj, ii);
+        const double rd = a.q[m], ru = a.q[m + np], rv = a.q[m + 2 * np], rw = a.q[m + 3 * np];
+        Ss[w] = rd;
+        Ss[ns + w] = ru;
+        Ss[2 * ns + w] = rv;
+        Ss[3 * ns + w] = rw;
+        Ss[4 * ns + w] = a.q[m + 4 * np];
+        Ss[5 * ns + w] = a.p[m];
+        if (LES) {
+          // the expressions of the four-point flux (double, or the FP32-bulk split form)
+          if constexpr (sizeof(T) == 4)
+            Ss[6 * ns + w] = (mt[m] * ru + mt[m + np] * rv + mt[m + 2 * np] * rw) * inv_refined(rd);
+          else
+            Ss[6 * ns + w] = 1. / rd * (mt[m] * ru + mt[m + np] * rv + mt[m + 2 * np] * rw);
+        }
+      }
+      __syncthreads();
+    }
     // ---- convective fluxes of direction d -> LDS
     for (int f = t; f < nfd; f += NT) {
       const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
       const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
       if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
+      Sv sv{nullptr, ns, 0, 0, false};
+      if (STG) {
+        sv.on = true;
+        sv.S = Ss;
+        sv.o0 = ((fk - oz) * by + (fj - oy)) * bx + (fi - ox);
+        sv.st = d == 0 ? 1 : d == 1 ? bx : bx * by;
+      }
       double F[5];
-      if (LES) conv_face_les<T>(a, d, fk, fj, fi, F);
-      else conv_face_plain<T>(a, d, fk, fj, fi, F);
+      if (LES) conv_face_les<T>(a, sv, d, fk, fj, fi, F);
+      else conv_face_plain<T>(a, sv, d, fk, fj, fi, F);
 #pragma unroll
       for (int q = 0; q < 5; q++) Fs[q][f] = F[q];
     }
@@ -747,21 +848,21 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     const T e = u0 * E0 + u1 * E1 + u2 * E2;
     visc[4] = visc[4] + double(e);
   }
-  if (!valid) return;
-
   // ---- phase D2: Rhs = aj (Visc - Conv) (MultAj, rhs.c:4611-4629)
-  double rhs[5];
-  const bool fluid = !a.has_ib || a.nv[n] < 0.1;
-  const double ajn = a.aj[n];
+  double rhs[5] = {0., 0., 0., 0., 0.}, qin[5] = {0., 0., 0., 0., 0.};
+  if (valid) {
+    const bool fluid = !a.has_ib || a.nv[n] < 0.1;
+    const double ajn = a.aj[n];
 #pragma unroll
-  for (int q = 0; q < 5; q++) {
-    double r = visc[q] - conv[q];
-    rhs[q] = fluid ? r * ajn : 0.;
+    for (int q = 0; q < 5; q++) {
+      double r = visc[q] - conv[q];
+      rhs[q] = fluid ? r * ajn : 0.;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) qin[q] = a.q[n + q * np];
   }
-  double qin[5];
-#pragma unroll
-  for (int q = 0; q < 5; q++) qin[q] = a.q[n + q * np];
   if (a.norms) {
+    // every lane of the wave takes part; lanes without a node contribute 0
     double mr = 0., mq = 0.;
 #pragma unroll
     for (int q = 0; q < 5; q++) {
@@ -776,6 +877,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       atomicMax(&a.norms[NSLOT + slot], dbits(mq));
     }
   }
+  if (!valid) return;
   if (MODE == 0) {
 #pragma unroll
     for (int q = 0; q < 5; q++) a.rhs[n + q * np] = rhs[q];
@@ -905,6 +1007,8 @@ int stage_batch_args(cles_ctx *c, int in, int out, int istage, const double coef
     case 1: tile_counts<TileB>(a.g, a); break;
     case 2: tile_counts<TileC>(a.g, a); break;
     case 3: tile_counts<TileD>(a.g, a); break;
+    case 4: tile_counts<TileE>(a.g, a); break;
+    case 5: tile_counts<TileF>(a.g, a); break;
     default: tile_counts<TileA>(a.g, a); break;
   }
   memcpy(dst, &a, sizeof(a));
@@ -918,6 +1022,8 @@ hipError_t launch_stage_batch(int tile, int fp32, int visc, const void *tab, con
     case 1: return launch_batch_shape<TileB>(fp32, visc, t, tstart, nreq, nblocks, s);
     case 2: return launch_batch_shape<TileC>(fp32, visc, t, tstart, nreq, nblocks, s);
     case 3: return launch_batch_shape<TileD>(fp32, visc, t, tstart, nreq, nblocks, s);
+    case 4: return launch_batch_shape<TileE>(fp32, visc, t, tstart, nreq, nblocks, s);
+    case 5: return launch_batch_shape<TileF>(fp32, visc, t, tstart, nreq, nblocks, s);
     default: return launch_batch_shape<TileA>(fp32, visc, t, tstart, nreq, nblocks, s);
   }
 }
@@ -930,6 +1036,8 @@ int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const doubl
     case 1: e = launch_shape<TileB>(c->p, a, mode, c->stream); break;
     case 2: e = launch_shape<TileC>(c->p, a, mode, c->stream); break;
     case 3: e = launch_shape<TileD>(c->p, a, mode, c->stream); break;
+    case 4: e = launch_shape<TileE>(c->p, a, mode, c->stream); break;
+    case 5: e = launch_shape<TileF>(c->p, a, mode, c->stream); break;
     default: e = launch_shape<TileA>(c->p, a, mode, c->stream); break;
   }
   CLES_HIP(c, e);

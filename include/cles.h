@@ -201,7 +201,8 @@ int cles_batch_remove(cles_batch *b, cles_ctx *c);
 int cles_batch_size(const cles_batch *b);
 size_t cles_batch_used(const cles_batch *b);
 /* Stage-kernel tile shape of the batched launches: 0 32x4x2, 1 16x4x4,
- * 2 16x8x2, 3 64x2x2 (chosen by the autotuner of the serving loop). */
+ * 2 16x8x2, 3 64x2x2, 4 6x6x6, 5 8x5x5 (chosen by the autotuner of the
+ * serving loop). */
 int cles_batch_set_tile(cles_batch *b, int tile);
 /* Advance every member by nsteps time steps (one graph replay per step) and
  * wait for them. */

@@ -10,7 +10,7 @@ import pytest
 import chimera_comp_les_amd as cl
 import pyoracle
 from chimera_comp_les_amd import cases
-from parity_util import RHS_TOL, STATE_TOL, perturbed, rhs_err, small_vortex, state_err
+from parity_util import RHS_TOL, STATE_TOL, channel_case, perturbed, rhs_err, small_vortex, state_err
 
 pytestmark = pytest.mark.gpu
 
@@ -134,7 +134,6 @@ def test_norms_match():
     ((7, 7, 1, 1, 52, 45), 2, 2),
 ])
 def test_steps_channel_boundaries(bc, iso, les):
-    from parity_util import channel_case
     cfg, coor, Q = channel_case(bc=bc, iso=iso, les=les)
     o, g = _pair(cfg, coor, Q)
     o.step(3)
@@ -145,3 +144,33 @@ def test_steps_channel_boundaries(bc, iso, les):
     assert np.all(np.isfinite(Qc))
     assert state_err(Qg, Qc) <= STATE_TOL
     assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+def test_tile_shapes_agree(tile):
+    """Every stage-kernel tile shape gives the bits of the default shape, on
+    grids whose interiors leave partial tiles at the far edges (workgroups of
+    the 6x6x6 and 8x5x5 shapes also have threads without a node), and the
+    first shape is checked against the CPU restatement as well."""
+    runs = []
+    for make in (lambda: small_vortex(shape=(19, 11, 9)), lambda: channel_case(nx=13, ny=10, nz=11, les=2)):
+        cfg, coor, Q = make()
+        res = []
+        for t in (1, tile):
+            g = cl.Solver(cfg, coor)
+            g.set_tile(t)
+            g.set_state(Q)
+            for _ in range(3):
+                g.flow_step()
+            res.append(g.state() + (g.norms(),))
+            g.close()
+        (Qa, Pa, na), (Qb, Pb, nb) = res
+        assert np.array_equal(Qa, Qb) and np.array_equal(Pa, Pb)
+        assert np.array_equal(na, nb)
+        runs.append((cfg, coor, Q, Qb))
+    if tile == 0:
+        cfg, coor, Q, Qg = runs[0]
+        o = pyoracle.Oracle(cfg, coor)
+        o.set_state(Q)
+        o.step(3)
+        assert state_err(Qg, o.state()[0]) <= STATE_TOL
