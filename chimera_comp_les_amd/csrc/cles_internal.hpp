@@ -201,6 +201,8 @@ struct cles_ctx {
   double *grad;    // 7 planes: strain-rate components S11 S12 S13 S22 S23 S33, |S|
   double *ua;      // 3 planes: velocity held by lUcat after the gradient loop
   double *lm, *mm;
+  double *lmq, *mmq;  // LM/27, MM/27: the terms of the 27-point means
+  double *ivol;       // 1/aj, the smoothing weights
   cles::SigmaSpec sig;
   // reductions
   unsigned long long *dnorm;  // [2] bit patterns of non-negative doubles

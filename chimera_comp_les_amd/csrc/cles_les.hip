@@ -32,6 +32,8 @@ struct LesArgs {
   double *grad;  // 7 planes: S11 S12 S13 S22 S23 S33 (no trace removal) and |S|
   double *ua;    // 3 planes: lUcat after the gradient loop
   double *lm, *mm;
+  double *lmq, *mmq;    // LM/27, MM/27
+  const double *ivol;   // 1/aj
   double *cs, *cs_out, *nut;
   const double *fil2;   // per-node (pow(1/aj, 1/3))^2, formed once at setup
   const double *tfil2;  // per-node squared test-filter width
@@ -140,69 +142,123 @@ __device__ __forceinline__ long long seam_image(const Grid &g, int k, int j, int
   return -1;
 }
 
-__global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a) {
+// Test filter (integrate_testfilter, les.c:80-131, and the L_ij M_ij
+// contraction of les.c:505-887).  A workgroup owns an 8x8x4 tile of interior
+// nodes and stages the node data of the tile plus a one-node halo in LDS, in
+// two passes over one 8 x 600 buffer: first rho, rhoU, rhoV, rhoW and the lUcat
+// velocity, then the six strain-rate components and |S| as the filter sees
+// them (periodic face copies of lSx..lSz, les.c:316-377; zero on other
+// boundary nodes; |S| has no periodic copy), with rho kept.  Every running
+// sum still adds its 27 terms in the reference's [k][j][i] order; the weights
+// 1/(|dk|+1) * 1/(|dj|+1) * 1/(|di|+1) are powers of two, exact as products.
+constexpr int FTX = 8, FTY = 8, FTZ = 4;
+constexpr int FBX = FTX + 2, FBY = FTY + 2, FBZ = FTZ + 2, FNB = FBX * FBY * FBZ;
+static_assert(FTX * FTY * FTZ == 256, "one node per thread");
+
+__device__ __forceinline__ double filter_w(int R, int Qd, int Pd) {
+  return (R == 1 ? 1.0 : 0.5) * (Qd == 1 ? 1.0 : 0.5) * (Pd == 1 ? 1.0 : 0.5);
+}
+
+__global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a, int ntx, int nty, int ntiles, int per_xcd) {
+  __shared__ double B[8][FNB];
   const Grid &g = a.g;
   const long long np = g.np;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int nxi = g.mx - 2, nyi = g.my - 2;
-  if (t >= (long long)nxi * nyi * (g.mz - 2)) return;
-  const int i = 1 + (int)(t % nxi), j = 1 + (int)((t / nxi) % nyi), k = 1 + (int)(t / ((long long)nxi * nyi));
-  const long long n0 = nid(g, k, j, i);
-  if (nvat(a, n0) > 1.1) {
-    a.lm[n0] = 0;
-    a.mm[n0] = 0;
+  const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);  // contiguous tiles per XCD
+  if (tile >= ntiles) return;
+  const int i0 = 1 + (tile % ntx) * FTX, j0 = 1 + ((tile / ntx) % nty) * FTY, k0 = 1 + (tile / (ntx * nty)) * FTZ;
+  const int t = threadIdx.x;
+  const int lx = t % FTX, ly = (t / FTX) % FTY, lz = t / (FTX * FTY);
+  const int i = i0 + lx, j = j0 + ly, k = k0 + lz;
+  const bool valid = i <= g.mx - 2 && j <= g.my - 2 && k <= g.mz - 2;
+  const long long n0 = valid ? nid(g, k, j, i) : 0;
+  const bool act = valid && !(nvat(a, n0) > 1.1);
+  const int c0 = ((lz + 1) * FBY + (ly + 1)) * FBX + (lx + 1);  // slot of the node itself
+  // ---- pass 1: density, momentum, velocity
+  for (int w = t; w < FNB; w += 256) {
+    const int ii = i0 - 1 + w % FBX, jj = j0 - 1 + (w / FBX) % FBY, kk = k0 - 1 + w / (FBX * FBY);
+    if (ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
+    const long long m = nid(g, kk, jj, ii);
+    B[0][w] = a.q[m];
+    B[1][w] = a.q[m + np];
+    B[2][w] = a.q[m + 2 * np];
+    B[3][w] = a.q[m + 3 * np];
+    B[4][w] = a.ua[m];
+    B[5][w] = a.ua[m + np];
+    B[6][w] = a.ua[m + 2 * np];
+  }
+  __syncthreads();
+  double s_rho = 0, s_ru = 0, s_rv = 0, s_rw = 0, s_ruu = 0, s_ruv = 0, s_ruw = 0, s_rvv = 0, s_rvw = 0, s_rww = 0;
+  double wsum = 0;
+  if (act) {
+#pragma unroll 1
+    for (int R = 0; R < 3; R++)
+#pragma unroll
+      for (int Qd = 0; Qd < 3; Qd++)
+#pragma unroll
+        for (int Pd = 0; Pd < 3; Pd++) {
+          const int o = c0 + ((R - 1) * FBY + (Qd - 1)) * FBX + (Pd - 1);
+          const double w = filter_w(R, Qd, Pd);
+          const double rho = B[0][o], ru = B[1][o], rv = B[2][o], rw = B[3][o];
+          const double u0 = B[4][o], u1 = B[5][o], u2 = B[6][o];
+          s_rho += w * rho;
+          s_ru += w * ru;
+          s_rv += w * rv;
+          s_rw += w * rw;
+          s_ruu += w * (ru * u0);
+          s_ruv += w * (ru * u1);
+          s_ruw += w * (ru * u2);
+          s_rvv += w * (rv * u1);
+          s_rvw += w * (rv * u2);
+          s_rww += w * (rw * u2);
+          wsum += w;
+        }
+  }
+  __syncthreads();
+  // ---- pass 2: strain rates and |S| as the filter sees them (rho stays in B[0])
+  for (int w = t; w < FNB; w += 256) {
+    const int ii = i0 - 1 + w % FBX, jj = j0 - 1 + (w / FBX) % FBY, kk = k0 - 1 + w / (FBX * FBY);
+    if (ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
+    long long m = nid(g, kk, jj, ii);
+    double S = 0.0;
+    if (interior(g, kk, jj, ii)) {
+      S = a.grad[m + 6 * np];
+    } else {
+      m = seam_image(g, kk, jj, ii);
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) B[1 + c][w] = m >= 0 ? a.grad[m + c * np] : 0.0;
+    B[7][w] = S;
+  }
+  __syncthreads();
+  if (!act) {
+    if (valid) {
+      a.lm[n0] = 0;
+      a.mm[n0] = 0;
+      a.lmq[n0] = 0;
+      a.mmq[n0] = 0;
+    }
     return;
   }
-  // running sums of the 27-point weighted means (integrate_testfilter,
-  // les.c:80-131) in the reference's [k][j][i] summation order; S_ab is
-  // symmetric, so S21/S31/S32 reuse the S12/S13/S23 sums (identical operands)
-  double s_rho = 0, s_ru = 0, s_rv = 0, s_rw = 0, s_ruu = 0, s_ruv = 0, s_ruw = 0, s_rvv = 0, s_rvw = 0, s_rww = 0;
   double sS[6] = {0, 0, 0, 0, 0, 0}, sSS[6] = {0, 0, 0, 0, 0, 0};
-  double wsum = 0;
-  double Sx[6];
 #pragma unroll 1
   for (int R = 0; R < 3; R++)
-#pragma unroll 1
+#pragma unroll
     for (int Qd = 0; Qd < 3; Qd++)
 #pragma unroll
       for (int Pd = 0; Pd < 3; Pd++) {
-        const int kk = k + R - 1, jj = j + Qd - 1, ii = i + Pd - 1;
-        const long long n = nid(g, kk, jj, ii);
-        const double w = 1. / (fabs(R - 1.0) + 1) * 1. / (fabs(Qd - 1.0) + 1) * 1. / (fabs(Pd - 1.0) + 1);
-        const double u0 = a.ua[n], u1 = a.ua[n + np], u2 = a.ua[n + 2 * np];
-        const double rho = a.q[n], ru = a.q[n + np], rv = a.q[n + 2 * np], rw = a.q[n + 3 * np];
-        double Sv[6], S;
-        if (interior(g, kk, jj, ii)) {
-#pragma unroll
-          for (int c = 0; c < 6; c++) Sv[c] = a.grad[n + c * np];
-          S = a.grad[n + 6 * np];
-        } else {
-          const long long s = seam_image(g, kk, jj, ii);
-#pragma unroll
-          for (int c = 0; c < 6; c++) Sv[c] = s >= 0 ? a.grad[s + c * np] : 0.0;
-          S = 0.0;  // |S| has no periodic copy (lS, les.c:225-330)
-        }
-        if (R == 1 && Qd == 1 && Pd == 1) {
-#pragma unroll
-          for (int c = 0; c < 6; c++) Sx[c] = Sv[c];
-        }
-        s_rho += w * rho;
-        s_ru += w * ru;
-        s_rv += w * rv;
-        s_rw += w * rw;
-        s_ruu += w * (ru * u0);
-        s_ruv += w * (ru * u1);
-        s_ruw += w * (ru * u2);
-        s_rvv += w * (rv * u1);
-        s_rvw += w * (rv * u2);
-        s_rww += w * (rw * u2);
+        const int o = c0 + ((R - 1) * FBY + (Qd - 1)) * FBX + (Pd - 1);
+        const double w = filter_w(R, Qd, Pd);
+        const double rho = B[0][o], S = B[7][o];
 #pragma unroll
         for (int c = 0; c < 6; c++) {
-          sS[c] += w * Sv[c];
-          sSS[c] += w * (Sv[c] * S * rho);
+          const double Sv = B[1 + c][o];
+          sS[c] += w * Sv;
+          sSS[c] += w * (Sv * S * rho);
         }
-        wsum += w;
       }
+  double Sx[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) Sx[c] = B[1 + c][c0];
   const double _rho = s_rho / wsum, _rhou = s_ru / wsum, _rhov = s_rv / wsum, _rhow = s_rw / wsum;
   // full 3x3 tensors in row-major order from the six symmetric components
   const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
@@ -239,6 +295,9 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a) {
     }
   a.lm[n0] = L_cat;
   a.mm[n0] = M_cat;
+  // the terms of the 27-point means of les_cs_kernel, each divided once here
+  a.lmq[n0] = L_cat / 27.0;
+  a.mmq[n0] = M_cat / 27.0;
 }
 
 // <LM>/<MM> and clipping on every owned node (les.c:889-930, 1146-1161)
@@ -262,8 +321,8 @@ __global__ __launch_bounds__(256) void les_cs_kernel(LesArgs a) {
           for (int b = -1; b <= 1; b++)
             for (int aa = -1; aa <= 1; aa++) {
               const long long m = nid(g, k + c, j + b, i + aa);
-              LM_avg += a.lm[m] / 27.0;
-              MM_avg += a.mm[m] / 27.0;
+              LM_avg += a.lmq[m];  // lm[m] / 27.0 (zero on boundary nodes, never written)
+              MM_avg += a.mmq[m];
             }
       }
       const double C = LM_avg / (MM_avg + LES_EPS);
@@ -301,7 +360,7 @@ __global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs a) {
   for (int c = -1; c <= 1; c++)
     for (int b = -1; b <= 1; b++)
       for (int aa = -1; aa <= 1; aa++) {
-        const double w = 1. / a.aj[nid(g, k + c, j + b, i + aa)];
+        const double w = a.ivol[nid(g, k + c, j + b, i + aa)];  // 1. / aj
         vs += w * lcs_at(a, k + c, j + b, i + aa);
         ws += w;
       }
@@ -352,13 +411,12 @@ int launch_les(cles_ctx *c) {
   a.qs = c->ucat_src >= 0 ? c->Q[c->ucat_src] : nullptr;
   a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert;
   a.has_ib = c->has_ib;
-  a.grad = c->grad; a.lm = c->lm; a.mm = c->mm; a.ua = c->ua; a.fil2 = c->fil2; a.tfil2 = c->tfil2;
+  a.grad = c->grad; a.lm = c->lm; a.mm = c->mm; a.lmq = c->lmq; a.mmq = c->mmq; a.ivol = c->ivol; a.ua = c->ua; a.fil2 = c->fil2; a.tfil2 = c->tfil2;
   a.cs = c->cs; a.cs_out = c->cs_alt; a.nut = c->nut;
   a.les = p.les; a.testfilter_ik = p.testfilter_ik; a.max_cs = p.max_cs;
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
-  const long long NI = (long long)(c->g.mx - 2) * (c->g.my - 2) * (c->g.mz - 2);
   const int bs = 256;
-  const unsigned gN = (unsigned)((N + bs - 1) / bs), gI = (unsigned)((NI + bs - 1) / bs);
+  const unsigned gN = (unsigned)((N + bs - 1) / bs);
   if (c->ucat_les_mixed)
     return set_error(c, CLES_ESTATE, "LES model called twice without a right-hand-side evaluation in between");
   const bool early = (c->ti < 2 && p.tistart == 0 && !p.rstart) || p.les == 10;
@@ -374,7 +432,12 @@ int launch_les(cles_ctx *c) {
     a.ucat_updated = 1;
     hipLaunchKernelGGL(les_grad_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
     CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(les_filter_kernel, dim3(gI), dim3(bs), 0, c->stream, a);
+    {
+      const int ntx = (c->g.mx - 2 + FTX - 1) / FTX, nty = (c->g.my - 2 + FTY - 1) / FTY,
+                ntz = (c->g.mz - 2 + FTZ - 1) / FTZ;
+      const int ntiles = ntx * nty * ntz, per_xcd = (ntiles + 7) / 8;
+      hipLaunchKernelGGL(les_filter_kernel, dim3(8 * per_xcd), dim3(256), 0, c->stream, a, ntx, nty, ntiles, per_xcd);
+    }
     CLES_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(les_cs_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
     CLES_HIP(c, hipGetLastError());
