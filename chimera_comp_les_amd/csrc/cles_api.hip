@@ -140,7 +140,7 @@ int check_params(const cles_params *p) {
 // squares are the exactly rounded products, as in the device kernels.
 int filter_widths(cles_ctx *c) {
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
-  std::vector<double> aj((size_t)N), f2((size_t)N), t2((size_t)N);
+  std::vector<double> aj((size_t)N), f2((size_t)N), t2((size_t)N), iv((size_t)N);
   int r = download(c, c->aj, aj.data(), 1, CLES_LAYOUT_GLOBAL);
   if (r) return r;
   const double c5 = pow(5.0, 1. / 3.);
@@ -149,8 +149,10 @@ int filter_widths(cles_ctx *c) {
     const double test_filter = c->p.testfilter_ik ? c5 * filter : 2.0 * filter;
     f2[(size_t)n] = filter * filter;
     t2[(size_t)n] = test_filter * test_filter;
+    iv[(size_t)n] = 1. / aj[(size_t)n];  // cell volume, the Cs smoothing weight (les.c:1301)
   }
   if ((r = upload(c, f2.data(), c->fil2, 1, CLES_LAYOUT_GLOBAL))) return r;
+  if (c->ivol && (r = upload(c, iv.data(), c->ivol, 1, CLES_LAYOUT_GLOBAL))) return r;
   return upload(c, t2.data(), c->tfil2, 1, CLES_LAYOUT_GLOBAL);
 }
 
@@ -178,8 +180,13 @@ int rk_coefficients(const cles_params &p, int istage, double coef[3], double *al
   return 0;
 }
 
+__global__ void zero_norms_kernel(unsigned long long *x) { x[threadIdx.x] = 0ull; }
+
+// the stage kernels' max slots start from zero (a fill kernel: no memset node
+// when a caller captures the stream)
 int reset_norms(cles_ctx *c) {
-  CLES_HIP(c, hipMemsetAsync(c->dnorm, 0, 128 * sizeof(unsigned long long), c->stream));
+  hipLaunchKernelGGL(zero_norms_kernel, dim3(1), dim3(128), 0, c->stream, c->dnorm);
+  CLES_HIP(c, hipGetLastError());
   return CLES_OK;
 }
 
@@ -197,7 +204,7 @@ size_t ctx_footprint(const cles_params *p) {
   size_t planes = 3 + 9 + 1 + 1 + 1 + 15 + 3 + 5;  // coor, metrics, aj, nvert, fil2, Q x3, P x3, rhs
   if (p->les) {
     planes += 1 + 3 + 3;  // tfil2, cs, cs_alt, nut, ua
-    if (p->les != 1 && p->les != 10) planes += 7 + 2;  // strain rates, LM, MM
+    if (p->les != 1 && p->les != 10) planes += 7 + 2 + 3;  // strain rates, LM, MM, LM/27, MM/27, 1/aj
   }
   const size_t xfer = (size_t)((p->mx + 4) * (long long)(p->my + 4) * (p->mz + 4)) * 5;
   return (planes * np + xfer) * sizeof(double);
@@ -280,6 +287,9 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
       TRY(dalloc(c, &c->grad, 7 * np));
       TRY(dalloc(c, &c->lm, np));
       TRY(dalloc(c, &c->mm, np));
+      TRY(dalloc(c, &c->lmq, np));
+      TRY(dalloc(c, &c->mmq, np));
+      TRY(dalloc(c, &c->ivol, np));
     }
   }
   {
@@ -301,7 +311,7 @@ void cles_destroy(cles_ctx *c) {
   if (!c) return;
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->ua, c->qsnap, c->tfil2};
+                    c->fil2, c->ua, c->qsnap, c->tfil2, c->lmq, c->mmq, c->ivol};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   ibm_free(c);
