@@ -26,6 +26,8 @@ struct BndArgs {
   int bc[6];
   int iso, twoD, has_ib;
   double gamma, Ma, Twall, dt, alfa;
+  double ren;  // Reynolds number (bctype 47 inflow)
+  int ti;      // time step of the stage (bctype 47 inflow)
   double a_rhs, a_old, a_cur;
   int istage;
   int periodic;  // any periodic direction
@@ -289,6 +291,36 @@ __global__ __launch_bounds__(256) void bc_k_kernel(BndArgs a, int ls, int le) {
   } else if (a.bc[4] == 10) {
     A.O(k, j, i, R) = 1.; A.O(k, j, i, U) = 0.; A.O(k, j, i, V) = 0.; A.O(k, j, i, W) = 1.;
     A.O(k, j, i, E) = 1. / 0.4 + 0.5;
+  } else if (a.bc[4] == 47) {
+    // power-law inflow profile plus travelling waves in x and time,
+    // rhs.c:904-945 (with its own pi = 3.141592); below y = 8 rhoU keeps the
+    // stage-updated value.  Operation order as written there.
+    const long long n = nid(g, k, j, i);
+    const double x = a.coor[n], yy = a.coor[n + g.np];
+    const double pw = 3.141592;
+    const double c1 = 0.03, c2 = -0.0032, c3 = 0.02, c4 = -0.01, y1 = 1.0, y2 = 1.5, y3 = 2.0;
+    const double w0 = 0.1, w1 = 0.25, w2 = 0.125, w3 = 0.0625, h1 = 0.0, h2 = 0.1, h3 = 0.15;
+    const double b0 = 0.5 * pw, b1 = 0.75 * pw, b2 = 0.5 * pw, b3 = 0.25 * pw;
+    const double yplus = yy * 0.065 * a.ren;
+    const int ti = a.ti;
+    double rw;
+    if (yy < 8.0) {
+      rw = pow(yy / 8.0, 0.1428);
+      rw += c1 * yplus * exp(-yplus / 12.5) * sin(w0 * ti * dt) * cos(b0 * x);
+      A.O(k, j, i, V) = c2 * yplus * yplus * exp(-yplus * yplus / 12.5 / 12.5) * sin(w0 * ti * dt) * cos(b0 * x);
+    } else {
+      rw = 1.0;
+      A.O(k, j, i, U) = 0.0;
+      rw += c3 * yy / y1 * exp(-yy / y1) * sin(w1 * ti * dt) * cos(b1 * x + h1) +
+            c3 * yy / y2 * exp(-yy / y2) * sin(w2 * ti * dt) * cos(b2 * x + h2) +
+            c3 * yy / y3 * exp(-yy / y3) * sin(w3 * ti * dt) * cos(b3 * x + h3);
+      A.O(k, j, i, V) = c4 * yy * yy / y1 / y1 * exp(-yy * yy / y1 / y1) * sin(w1 * ti * dt) * cos(b1 * x + h1) +
+                        c4 * yy * yy / y2 / y2 * exp(-yy * yy / y2 / y2) * sin(w2 * ti * dt) * cos(b2 * x + h2) +
+                        c4 * yy * yy / y3 / y3 * exp(-yy * yy / y3 / y3) * sin(w3 * ti * dt) * cos(b3 * x + h3);
+    }
+    A.O(k, j, i, W) = rw;
+    A.O(k, j, i, R) = 1.0;
+    A.O(k, j, i, E) = 0.5 * rw * rw + 1. / (Ma * Ma * gam * (gam - 1.));
   }
   k = g.mz - 1;
   if (a.bc[5] == 3) {
@@ -449,7 +481,7 @@ bool supported_bc(int side, int b) {
   if (b == 7 || b == 0) return true;
   if (side < 2) return b == 3 || b == 51;
   if (side < 4) return b == 3 || b == 10 || b == 1;
-  if (side == 4) return b == 3 || b == 52 || b == 55 || b == 44 || b == 45 || b == 51 || b == 10;
+  if (side == 4) return b == 3 || b == 52 || b == 55 || b == 44 || b == 45 || b == 51 || b == 10 || b == 47;
   return b == 3 || b == 44 || b == 45 || b == 49 || b == 51 || b == 10 || b == 8;
 }
 
@@ -469,6 +501,7 @@ int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const
   for (int q = 0; q < 6; q++) a.bc[q] = p.bctype[q];
   a.iso = p.isothermal; a.twoD = p.twoD; a.has_ib = c->has_ib;
   a.gamma = p.gamma; a.Ma = p.Ma; a.Twall = p.Twall; a.dt = p.dt; a.alfa = alfa;
+  a.ren = p.ren; a.ti = c->ti;
   a.a_rhs = coef[0]; a.a_old = coef[1]; a.a_cur = coef[2];
   a.istage = istage;
   a.periodic = c->g.px || c->g.py || c->g.pz;
@@ -515,6 +548,7 @@ int seam_batch_args(cles_ctx *c, int in, int out, void *dst) {
   a.aj = c->aj; a.eta = c->eta; a.zet = c->zet; a.nv = c->nvert; a.coor = c->coor;
   for (int q = 0; q < 6; q++) a.bc[q] = p.bctype[q];
   a.gamma = p.gamma; a.Ma = p.Ma; a.Twall = p.Twall; a.dt = p.dt;
+  a.ren = p.ren; a.ti = c->ti;
   a.periodic = 1;
   const long long mx = c->g.mx, my = c->g.my, mz = c->g.mz;
   a.nb = 2 * mx * my + 2 * mx * (mz - 2) + 2 * (my - 2) * (mz - 2);
@@ -541,6 +575,7 @@ int launch_boundary_post(cles_ctx *c, int buf) {
   for (int q = 0; q < 6; q++) a.bc[q] = p.bctype[q];
   a.iso = p.isothermal; a.twoD = p.twoD; a.has_ib = c->has_ib;
   a.gamma = p.gamma; a.Ma = p.Ma; a.Twall = p.Twall; a.dt = p.dt; a.alfa = 0.;
+  a.ren = p.ren; a.ti = c->ti;
   a.a_rhs = 0.; a.a_old = 0.; a.a_cur = 0.;
   a.istage = 0;
   a.periodic = c->g.px || c->g.py || c->g.pz;

@@ -419,11 +419,17 @@ int launch_les(cles_ctx *c) {
   const unsigned gN = (unsigned)((N + bs - 1) / bs);
   if (c->ucat_les_mixed)
     return set_error(c, CLES_ESTATE, "LES model called twice without a right-hand-side evaluation in between");
-  const bool early = (c->ti < 2 && p.tistart == 0 && !p.rstart) || p.les == 10;
+  // les.c:139 tests the global tistart, which main.c:890-891 has incremented
+  // before the time loop (a fresh run 0 -> 1, a restart from N -> N + 1): in
+  // the loop it is never 0, so the Cs = 0 early-out is taken for les == 10
+  // only, and the first step of a fresh run computes the dynamic model from
+  // an lUcat that no CalcUCat has written yet (zero)
+  const int tistart_in_loop = p.tistart + 1;
+  const bool early = (c->ti < 2 && tistart_in_loop == 0 && !p.rstart) || p.les == 10;
   a.ucat_updated = 0;
   if (early || p.les == 1) {
-    // les.c:139-151: Cs = 0 (first step of a fresh run) or 0.03 everywhere;
-    // lUcat keeps the velocity of the last CalcUCat
+    // les.c:139-151: Cs = 0 or 0.03 everywhere; lUcat keeps the velocity of
+    // the last CalcUCat
     hipLaunchKernelGGL(fill_kernel, dim3(gN), dim3(bs), 0, c->stream, c->cs, N, early ? 0.0 : 0.03);
     CLES_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(les_grad_kernel, dim3(gN), dim3(bs), 0, c->stream, a);

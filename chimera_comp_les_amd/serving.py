@@ -5,9 +5,11 @@ Requests (isentropic vortex or viscous Taylor-Green, sizes drawn from
 {32, 48, 64, 96, 128}^3, Mach uniform in [0.1, 0.5], Re log-uniform in
 [100, 5000], a fixed number of SSP-RK3 steps, FP32-bulk precision) arrive as a
 Poisson stream.  The loop keeps every admitted solve resident on the device,
-admits waiting requests whenever their HBM footprint fits the budget (also
-while others are in flight), advances all members together with the batched,
-graph-captured RK3 step, and retires each request when its steps are done.
+admits waiting requests (smallest work first, while the nodes in flight stay
+below a cap that keeps every CU busy, and the HBM footprint fits the budget),
+also while others are in flight, advances all members together with the
+batched, graph-captured RK3 step, and retires each request when its steps
+are done.
 The tile shape of the batched stage kernel comes from `autotune`, which times
 every shape on a sample of the mix.
 
@@ -80,15 +82,25 @@ def digest(s):
 
 
 def run_sequential(reqs, pay, precision=cl.PREC_FP32BULK):
-    """Solve the requests one at a time: set-up, all steps, download."""
+    """Solve the requests one at a time: set-up, all steps, download.  The
+    latency of a one-request-at-a-time server is formed from the measured
+    service times: requests are served in arrival order, each starting when
+    it has arrived and the previous one has finished."""
     t0 = time.perf_counter()
-    res = {}
+    res, service = {}, {}
     for r in reqs:
+        a = time.perf_counter()
         s = build(pay[r.rid], precision)
         s.advance(r.steps)
         res[r.rid] = digest(s)
         s.close()
-    return {"wall_s": time.perf_counter() - t0, "digests": res}
+        service[r.rid] = time.perf_counter() - a
+    wall = time.perf_counter() - t0
+    lat, end = {}, 0.0
+    for r in sorted(reqs, key=lambda q: q.arrival):
+        end = max(end, r.arrival) + service[r.rid]
+        lat[r.rid] = end - r.arrival
+    return {"wall_s": wall, "digests": res, "latency_s": lat}
 
 
 def autotune(sample, budget, nrep=5, precision=cl.PREC_FP32BULK):
@@ -114,13 +126,20 @@ def autotune(sample, budget, nrep=5, precision=cl.PREC_FP32BULK):
     return best, times
 
 
-def run_batched(reqs, pay, budget, tile, chunk=10, precision=cl.PREC_FP32BULK):
+def run_batched(reqs, pay, budget, tile, chunk=10, precision=cl.PREC_FP32BULK, policy="sjf", cap_cells=6 << 20):
     """Continuous batching in real time.  Each round queues `chunk` batched
     steps (fewer if a member finishes earlier) and, while they run, downloads
     the requests finished in the previous round and sets up newly arrived
-    ones; after the round, finished members leave and the new ones join."""
+    ones; after the round, finished members leave and the new ones join.
+
+    Admission: a request joins when its HBM footprint fits the budget and,
+    with policy "sjf", when the nodes in flight stay below `cap_cells`
+    (enough tiles to keep every CU busy); waiting requests are taken
+    smallest work first, so short requests do not queue behind long ones.
+    Policy "all" admits every arrived request that fits the budget, in
+    arrival order."""
     pending = collections.deque(sorted(reqs, key=lambda r: r.arrival))
-    waiting = collections.deque()
+    waiting = []
     active = {}           # Solver -> [request, steps left]
     leaving, closing, staged = [], [], []
     b = cl.Batch(budget)
@@ -143,14 +162,22 @@ def run_batched(reqs, pay, budget, tile, chunk=10, precision=cl.PREC_FP32BULK):
             lat[r.rid] = time.perf_counter() - t0 - r.arrival
             closing.append(s)
         leaving = []
+        if policy == "sjf":
+            waiting.sort(key=work)
+        inflight = sum(v[0].N ** 3 for v in active.values()) + sum(r.N ** 3 for _, r in staged)
         while waiting:
             r = waiting[0]
             need = cl.footprint(pay[r.rid][0], precision)
             if b.used() + held + need > budget:
+                if not (active or staged or leaving or closing):
+                    raise RuntimeError("request %d needs %d bytes, more than the budget" % (r.rid, need))
+                break
+            if policy == "sjf" and inflight and inflight + r.N ** 3 > cap_cells:
                 break
             staged.append((build(pay[r.rid], precision), r))
             held += need
-            waiting.popleft()
+            inflight += r.N ** 3
+            waiting.pop(0)
         if k:
             b.wait()
         for s in closing:   # freeing waits for an idle device: do it between rounds
@@ -189,6 +216,8 @@ def main(argv=None):
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--chunk", type=int, default=10, help="steps between admission/retirement points")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--policy", choices=["sjf", "all"], default="sjf")
+    ap.add_argument("--cap-mcells", type=float, default=6.0, help="nodes in flight (millions) under policy sjf")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     reqs = make_requests(args.requests, args.rate, args.steps, args.seed)
@@ -202,8 +231,10 @@ def main(argv=None):
           flush=True)
     seq = run_sequential(reqs, pay)
     print(json.dumps({"sequential_s": seq["wall_s"]}), flush=True)
-    bat = run_batched(reqs, pay, budget, tile, args.chunk)
+    cap = int(args.cap_mcells * 1e6)
+    bat = run_batched(reqs, pay, budget, tile, args.chunk, policy=args.policy, cap_cells=cap)
     lat = np.array([bat["latency_s"][r.rid] for r in reqs])
+    slat = np.array([seq["latency_s"][r.rid] for r in reqs])
     same = sum(seq["digests"][r.rid] == bat["digests"][r.rid] for r in reqs)
     out = {
         "requests": len(reqs), "steps": args.steps, "rate_per_s": args.rate, "precision": "fp32bulk",
@@ -211,7 +242,10 @@ def main(argv=None):
         "sequential_wall_s": seq["wall_s"], "sequential_cell_stage_per_s": total / seq["wall_s"],
         "batched_makespan_s": bat["makespan_s"], "batched_cell_stage_per_s": total / bat["makespan_s"],
         "speedup": seq["wall_s"] / bat["makespan_s"],
+        "policy": args.policy, "cap_cells": cap if args.policy == "sjf" else None,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+        "sequential_latency_p50_s": float(np.percentile(slat, 50)),
+        "sequential_latency_p99_s": float(np.percentile(slat, 99)),
         "mean_concurrency": bat["mean_concurrency"], "max_concurrency": bat["max_concurrency"],
         "identical_results": int(same),
     }
@@ -220,7 +254,8 @@ def main(argv=None):
         with open(args.out, "w") as f:
             f.write("# Serving: %d requests, %d steps, Poisson %.1f/s, FP32-bulk\n\n" % (len(reqs), args.steps, args.rate))
             f.write("Command: `python -m chimera_comp_les_amd.serving --requests %d --rate %g --steps %d "
-                    "--budget-gb %g --chunk %d`\n\n" % (len(reqs), args.rate, args.steps, args.budget_gb, args.chunk))
+                    "--budget-gb %g --chunk %d --policy %s --cap-mcells %g`\n\n"
+                    % (len(reqs), args.rate, args.steps, args.budget_gb, args.chunk, args.policy, args.cap_mcells))
             f.write("Autotuner (ms per batched step of one request per size): %s -> %s\n\n"
                     % (json.dumps({TILES[t]: round(v, 4) for t, v in ttimes.items()}), TILES[tile]))
             f.write("```\n%s\n```\n" % json.dumps(out, indent=1))

@@ -73,7 +73,8 @@ typedef struct cles_params {
   int abc;               /* -ABC                                              */
   int twoD;              /* -twoD (must be 0 on this path)                    */
   int testfilter_ik;     /* -testfilter_ik                                    */
-  int tistart;           /* first time step of the run (0 = fresh start)      */
+  int tistart;           /* -rstart value (0 = fresh start); the time loop
+                            sees it incremented (main.c:890-891)            */
   int rstart;            /* restart flag                                      */
   int precision;         /* CLES_PREC_*                                       */
 } cles_params;

@@ -497,6 +497,37 @@ void orc_form_bcs(orc_ctx *c, double alfa) {
         } else if (bc[4] == 10) {
           Qg(k, j, i).rho = 1.; Qg(k, j, i).rhoU = 0.; Qg(k, j, i).rhoV = 0.;
           Qg(k, j, i).rhoW = 1.; Qg(k, j, i).rhoE = 1. / 0.4 + 0.5;
+        } else if (bc[4] == 47) {
+          /* power-law inflow with travelling-wave disturbances, rhs.c:904-945;
+           * rhoU is left as the stage update made it below y = 8 */
+          const orc_v3 *co = &c->coor[GI(c, k, j, i)];
+          const double pw = 3.141592;
+          const double c1 = 0.03, c2 = -0.0032, c3 = 0.02, c4 = -0.01, yp1 = 1.0, yp2 = 1.5, yp3 = 2.0;
+          const double w0 = 0.1, w1 = 0.25, w2 = 0.125, w3 = 0.0625;
+          const double phi1 = 0.0, phi2 = 0.1, phi3 = 0.15;
+          const double b0 = 0.5 * pw, b1 = 0.75 * pw, b2 = 0.5 * pw, b3 = 0.25 * pw;
+          const double yy = co->y, yplus = yy * 0.065 * P->ren;
+          if (yy < 8.0) {
+            Qg(k, j, i).rhoW = pow(yy / 8.0, 0.1428);
+          } else {
+            Qg(k, j, i).rhoW = 1.0;
+            Qg(k, j, i).rhoU = 0.0;
+          }
+          if (yy < 8.0) {
+            Qg(k, j, i).rhoW += c1 * yplus * exp(-yplus / 12.5) * sin(w0 * c->ti * dt) * cos(b0 * co->x);
+            Qg(k, j, i).rhoV = c2 * yplus * yplus * exp(-yplus * yplus / 12.5 / 12.5) * sin(w0 * c->ti * dt) *
+                               cos(b0 * co->x);
+          } else {
+            Qg(k, j, i).rhoW += c3 * yy / yp1 * exp(-yy / yp1) * sin(w1 * c->ti * dt) * cos(b1 * co->x + phi1) +
+                                c3 * yy / yp2 * exp(-yy / yp2) * sin(w2 * c->ti * dt) * cos(b2 * co->x + phi2) +
+                                c3 * yy / yp3 * exp(-yy / yp3) * sin(w3 * c->ti * dt) * cos(b3 * co->x + phi3);
+            Qg(k, j, i).rhoV =
+                c4 * yy * yy / yp1 / yp1 * exp(-yy * yy / yp1 / yp1) * sin(w1 * c->ti * dt) * cos(b1 * co->x + phi1) +
+                c4 * yy * yy / yp2 / yp2 * exp(-yy * yy / yp2 / yp2) * sin(w2 * c->ti * dt) * cos(b2 * co->x + phi2) +
+                c4 * yy * yy / yp3 / yp3 * exp(-yy * yy / yp3 / yp3) * sin(w3 * c->ti * dt) * cos(b3 * co->x + phi3);
+          }
+          Qg(k, j, i).rho = 1.0;
+          Qg(k, j, i).rhoE = 0.5 * Qg(k, j, i).rhoW * Qg(k, j, i).rhoW + 1. / (Ma * Ma * gam * (gam - 1.));
         }
         k = mz - 1;
         if (bc[5] == 3) {

@@ -185,7 +185,11 @@ static double integ_t(const double v[27]) { /* integrate_t, les.c:66-78 */
 static void smagorinsky(orc_ctx *c) {
   const orc_params *P = &c->p;
   const int mx = c->mx, my = c->my, mz = c->mz;
-  if ((c->ti < 2 && P->tistart == 0 && !P->rstart) || P->les == 10) {
+  /* les.c:139 reads the global tistart after main.c:890-891 has incremented
+   * it (0 -> 1 for a fresh run, N -> N+1 for a restart): never 0 in the time
+   * loop, so only les == 10 takes this branch */
+  const int tistart_in_loop = P->tistart + 1;
+  if ((c->ti < 2 && tistart_in_loop == 0 && !P->rstart) || P->les == 10) {
     for (size_t n = 0; n < c->nG; n++) c->CS[n] = 0.0;
     return;
   }

@@ -56,14 +56,15 @@ def perturbed(Q, seed, amp=1e-3):
     return cases.fill_seams(Q * f)
 
 
-def channel_case(nx=8, ny=10, nz=12, bc=(7, 7, 1, 1, 45, 8), iso=0, les=0, invicid=0, Ma=0.3, seed=0):
-    """Stretched channel: periodic in i, walls in j, inflow/outflow in k (the only
-    orientation the reference supports), uniform flow along k with a seeded
-    perturbation."""
+def channel_case(nx=8, ny=10, nz=12, bc=(7, 7, 1, 1, 45, 8), iso=0, les=0, invicid=0, Ma=0.3, seed=0,
+                 height=1.0):
+    """Stretched channel of the given height: periodic in i, walls in j,
+    inflow/outflow in k (the only orientation the reference supports), uniform
+    flow along k with a seeded perturbation."""
     rng = np.random.default_rng(seed)
     xi = (np.arange(nx + 2) - 1.0) / nx
     eta = np.linspace(0.0, 1.0, ny + 2)
-    y = 0.5 * (1 + np.tanh(2.2 * (eta - 0.5)) / np.tanh(1.1))   # tanh clustering at the walls
+    y = height * 0.5 * (1 + np.tanh(2.2 * (eta - 0.5)) / np.tanh(1.1))   # tanh clustering at the walls
     z = np.linspace(0.0, 2.0, nz + 2)
     Z, Y, X = np.meshgrid(z, y, xi, indexing="ij")
     coor = np.stack([X, Y, Z], axis=-1)

@@ -62,8 +62,8 @@ def _create_err(**kw):
 
 def test_create_rejects_unsupported_options_before_device_use():
     # parameter validation runs on the host before any allocation
-    e, msg = _create_err(bctype=(7, 7, 1, 1, 47, 8))
-    assert e == -1 and "bctype[4]=47" in msg
+    e, msg = _create_err(bctype=(7, 7, 1, 1, 49, 8))   # 49 is a k-max type only (rhs.c:973)
+    assert e == -1 and "bctype[4]=49" in msg
     e, msg = _create_err(bctype=(7, 3, 7, 7, 7, 7))
     assert e == -1 and "periodic" in msg
     e, msg = _create_err(fluxorder=4)

@@ -65,9 +65,9 @@ def test_fp32bulk_channel_close_to_fp64():
     assert e[0] < 1e-8 and e[4] < 1e-8, e
 
 
-def _case(case, fluxorder=3):
+def _case(case, fluxorder=3, les=2):
     if case == "tgv":
-        cfg, coor, Q = cases.tgv_case(N=32, fluxorder=fluxorder)
+        cfg, coor, Q = cases.tgv_case(N=32, fluxorder=fluxorder, les=les)
     else:
         cfg, coor, Q = small_vortex(N=24)
     cfg = dict(cfg)
@@ -84,7 +84,10 @@ def test_single_stage_conserves_to_roundoff(case, prec):
     reference copies seams from the stage input, so later stages see lagged
     seam nodes and the two periodic boundary faces no longer carry the same
     flux; the drop-in keeps that behaviour, see the oracle comparison below.)"""
-    cfg, coor, Q = _case(case, fluxorder=1)
+    # viscous TGV without the LES model: the reference computes nu_t on interior
+    # nodes only (les.c:1536-1573), so with the model the SGS flux of the two
+    # periodic boundary faces differs and the LES form does not telescope
+    cfg, coor, Q = _case(case, fluxorder=1, les=0)
     s = cl.Solver(cfg, coor, precision=prec)
     s.set_state(Q)
     i0 = s.integrals()

@@ -100,7 +100,7 @@ def test_steps_tgv_les(fluxorder):
     cfg, coor, Q = cases.tgv_case(N=12, fluxorder=fluxorder)
     Q = perturbed(Q, 5, amp=1e-2)
     o, g = _pair(cfg, coor, Q)
-    for s in range(4):  # step 1 has Cs = 0; steps 2-4 run the dynamic model
+    for s in range(4):  # step 1: dynamic model from a zero lUcat; steps 2-4 from the stage-3 input
         o.step(1)
         g.flow_step()
         cso, nuo, _ = o.les_fields()
@@ -142,6 +142,25 @@ def test_steps_channel_boundaries(bc, iso, les):
     Qg, Pg = g.state()
     g.close()
     assert np.all(np.isfinite(Qc))
+    assert state_err(Qg, Qc) <= STATE_TOL
+    assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL
+
+
+@pytest.mark.parametrize("les", [0, 2])
+def test_steps_synthetic_inflow(les):
+    """k-min bctype 47 (rhs.c:904-945): power-law profile with travelling
+    waves, both the y < 8 and the y >= 8 branch (channel height 12); the
+    profile uses libm exp/sin/cos/pow on the CPU and the device library's on
+    the GPU, so the bar is the tolerance, not bits."""
+    cfg, coor, Q = channel_case(bc=(7, 7, 1, 1, 47, 8), les=les, height=12.0)
+    assert coor[..., 1].max() > 8.0 > coor[..., 1].min()
+    o, g = _pair(cfg, coor, Q)
+    o.step(3)
+    g.advance(3)
+    Qc, Pc = o.state()
+    Qg, Pg = g.state()
+    g.close()
+    assert np.any(Qc[0, :, :, 2] != Q[0, :, :, 2])  # the inflow plane was rewritten
     assert state_err(Qg, Qc) <= STATE_TOL
     assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL
 

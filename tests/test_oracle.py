@@ -219,7 +219,7 @@ def test_les_gradient_two_array_form_equals_the_literal_loop():
     assert np.array_equal(A1, A2) and np.array_equal(S1, S2)
 
 
-def _shear_case(a, max_cs=10.0):
+def _shear_case(a, max_cs=10.0, calc_ucat=True):
     """u = a (x + y) on a Cartesian grid, rho = 1, uniform p (deep interior only)."""
     n = 14
     coor, h = cases.cartesian_grid(n, L=n * 0.125)
@@ -230,8 +230,9 @@ def _shear_case(a, max_cs=10.0):
                max_cs=max_cs, dt=1e-4)
     o = pyoracle.Oracle(cfg, coor)
     o.set_state(Q)
-    o.form_rhs(0)  # CalcUCat: the "previous" velocity equals the current one
-    o.ti = 5       # past the fresh-start Cs = 0 steps
+    if calc_ucat:
+        o.form_rhs(0)  # CalcUCat: the "previous" velocity equals the current one
+    o.ti = 5
     return o, h
 
 
@@ -265,9 +266,17 @@ def test_dynamic_model_clips():
     assert np.allclose(cs[(slice(4, -4),) * 3], 1e-4)      # PetscMax(C, 0.0001) (les.c:924)
 
 
-def test_dynamic_model_first_step_of_a_fresh_run_is_zero():
-    o, h = _shear_case(0.8)
-    o.ti = 1  # ti < 2 with tistart == 0: Cs = 0 (les.c:139-142)
-    o.les_update()
-    cs, nut, _ = o.les_fields()
-    assert np.all(cs == 0) and np.all(nut == 0)
+def test_dynamic_model_runs_from_the_first_step():
+    """les.c:139 tests the global tistart, which main.c:890-891 increments
+    before the time loop, so in the loop the Cs = 0 early-out never fires: the
+    first step (ti = 1) computes the dynamic model like any later step, from
+    an lUcat no CalcUCat has written yet (zero)."""
+    res = []
+    for ti in (1, 5):
+        o, h = _shear_case(0.8, calc_ucat=False)
+        o.ti = ti
+        o.les_update()
+        res.append(o.les_fields()[:2])
+    (cs1, nut1), (cs5, nut5) = res
+    assert np.array_equal(cs1, cs5) and np.array_equal(nut1, nut5)
+    assert np.any(cs1 != 0) and np.any(nut1 != 0)
