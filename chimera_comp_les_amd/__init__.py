@@ -63,6 +63,7 @@ EXPORTS = [
     "cles_batch_create", "cles_batch_destroy", "cles_batch_add", "cles_batch_remove", "cles_batch_size",
     "cles_batch_used", "cles_batch_set_tile", "cles_batch_advance", "cles_batch_time_step",
     "cles_batch_last_error", "cles_set_tile", "cles_batch_launch", "cles_batch_wait",
+    "cles_set_state_qp", "cles_write_fields", "cles_read_fields", "cles_petsc_vec_write", "cles_petsc_vec_read",
 ]
 
 IBM_INFO_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
@@ -142,6 +143,11 @@ def load_library(path=LIB_PATH):
     L.cles_ibm_search.argtypes = [vp, C.c_int, dp, dp, dp, C.c_int, ip, ip, ip, dp, ip]
     L.cles_ibm_set_velocity.argtypes = [vp, dp, dp]
     L.cles_ibm_get_list.argtypes = [vp, C.POINTER(CIbmInfo), C.c_int]
+    L.cles_set_state_qp.argtypes = [vp, dp, dp, C.c_int]
+    L.cles_write_fields.argtypes = [vp, C.c_char_p, C.c_int]
+    L.cles_read_fields.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
+    L.cles_petsc_vec_write.argtypes = [C.c_char_p, dp, C.c_longlong, C.c_int]
+    L.cles_petsc_vec_read.argtypes = [C.c_char_p, dp, C.c_longlong]
     _LIB = L
     return L
 
@@ -216,6 +222,22 @@ class Solver:
     def set_state(self, Q, layout=LAYOUT_GLOBAL):
         Q = np.ascontiguousarray(Q, dtype=np.float64)
         self._check(self.L.cles_set_state(self.h, _ptr(Q), layout))
+
+    def set_state_qp(self, Q, P, layout=LAYOUT_GLOBAL):
+        """Q and P as given (no EqOfState); Qold = Q."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        self._check(self.L.cles_set_state_qp(self.h, _ptr(Q), _ptr(P), layout))
+
+    def write_fields(self, directory=None, block=0):
+        """Q_P_Binary_Output: the reference's binary field files of the current step."""
+        d = None if directory is None else os.fsencode(directory)
+        self._check(self.L.cles_write_fields(self.h, d, int(block)))
+
+    def read_fields(self, ti, directory=None, block=0):
+        """Q_P_Binary_Input + RestartApplication from the files of step ti."""
+        d = None if directory is None else os.fsencode(directory)
+        self._check(self.L.cles_read_fields(self.h, d, int(ti), int(block)))
 
     def state(self, layout=LAYOUT_GLOBAL):
         if layout == LAYOUT_GLOBAL:
@@ -333,8 +355,26 @@ class Solver:
         return out
 
     def set_tile(self, tile):
-        """Stage-kernel tile shape: 0 32x4x2, 1 16x4x4, 2 16x8x2, 3 64x2x2, 4 6x6x6, 5 8x5x5."""
+        """Stage-kernel tile shape: 0 32x4x2, 1 16x4x4, 2 16x8x2, 3 64x2x2, 4 6x6x6, 5 8x5x5
+        (256 threads), 6 4x4x3 (64 threads), 7 8x4x3 (128 threads)."""
         self._check(self.L.cles_set_tile(self.h, int(tile)))
+
+
+def petsc_vec_write(path, x, block_size=1):
+    """Write x (flattened, float64) as one PETSc binary Vec file."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    L = load_library()
+    if L.cles_petsc_vec_write(os.fsencode(path), _ptr(x), x.size, int(block_size)) != 0:
+        raise CLESError(L.cles_last_error(None).decode())
+
+
+def petsc_vec_read(path, n):
+    """Read a PETSc binary Vec file of exactly n values."""
+    x = np.empty(int(n))
+    L = load_library()
+    if L.cles_petsc_vec_read(os.fsencode(path), _ptr(x), x.size) != 0:
+        raise CLESError(L.cles_last_error(None).decode())
+    return x
 
 
 def footprint(cfg, precision=None):

@@ -342,14 +342,16 @@ int cles_get_nvert(cles_ctx *c, double *nvert) {
   return download(c, c->nvert, nvert, 1, CLES_LAYOUT_GLOBAL);
 }
 
-int cles_set_state(cles_ctx *c, const double *Q, int layout) {
+// Q (and P, or EqOfState when P is null) into a free state buffer, which
+// becomes Q and Qold
+static int set_state(cles_ctx *c, const double *Q, const double *P, int layout) {
   if (!c || !Q) return CLES_EINVAL;
   // keep the buffer that holds the last CalcUCat input intact
   const int b = c->ucat_src == c->cur ? free_buffer(c, c->cur, -1) : c->cur;
   int r = upload(c, Q, c->Q[b], 5, layout);
   if (r) return r;
   c->cur = b;
-  r = launch_eos_all(c, b);
+  r = P ? upload(c, P, c->P[b], 1, layout) : launch_eos_all(c, b);
   if (r) return r;
   CLES_HIP(c, hipStreamSynchronize(c->stream));
   if (c->p.precision == CLES_PREC_FP32BULK) {
@@ -361,6 +363,13 @@ int cles_set_state(cles_ctx *c, const double *Q, int layout) {
   }
   c->have_state = 1;
   return CLES_OK;
+}
+
+int cles_set_state(cles_ctx *c, const double *Q, int layout) { return set_state(c, Q, nullptr, layout); }
+
+int cles_set_state_qp(cles_ctx *c, const double *Q, const double *P, int layout) {
+  if (!P) return CLES_EINVAL;
+  return set_state(c, Q, P, layout);
 }
 
 int cles_get_state(cles_ctx *c, double *Q, double *P, int layout) {
@@ -601,7 +610,7 @@ int cles_time_les(cles_ctx *c, int nrep, double *ms) {
 size_t cles_device_bytes(const cles_ctx *c) { return c ? c->bytes : 0; }
 
 int cles_set_tile(cles_ctx *c, int tile) {
-  if (!c || tile < 0 || tile > 5) return CLES_EINVAL;
+  if (!c || tile < 0 || tile > 7) return CLES_EINVAL;
   c->tile = tile;
   return CLES_OK;
 }

@@ -226,7 +226,7 @@ int cles_batch_size(const cles_batch *b) { return b ? (int)b->members.size() : C
 size_t cles_batch_used(const cles_batch *b) { return b ? b->used : 0; }
 
 int cles_batch_set_tile(cles_batch *b, int tile) {
-  if (!b || tile < 0 || tile > 5) return CLES_EINVAL;
+  if (!b || tile < 0 || tile > 7) return CLES_EINVAL;
   if (tile != b->tile) {
     b->tile = tile;
     b->dirty = true;

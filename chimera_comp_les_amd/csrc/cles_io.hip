@@ -1,0 +1,157 @@
+// cles_io.hip -- the reference's binary field files: Q_P_Binary_Output and
+// Q_P_Binary_Input (main.c:215-304) with the restart of RestartApplication
+// (main.c:621-650).
+//
+// Each field is one PETSc binary Vec file as VecView writes it for a
+// single-rank DMDA vector (PETSc 3.12, 32-bit PetscInt): a big-endian int32
+// class id 1211214 (VEC_FILE_CLASSID), a big-endian int32 length, then the
+// values as big-endian doubles in the natural ordering (i fastest, the dof of
+// a node interleaved).  VecView also writes `<file>.info` with
+// "-vecload_block_size <dof>" for dof > 1.  File names follow the reference:
+// Qfield%05d_%1d.dat, pfield..., nvfield..., and with LES sigma... and
+// nu_t...; the number is the time step, the digit the block.
+#include <cerrno>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cles_internal.hpp"
+
+namespace cles {
+namespace {
+
+constexpr int32_t kVecClassId = 1211214;
+
+uint32_t be32(uint32_t x) { return __builtin_bswap32(x); }
+uint64_t be64(uint64_t x) { return __builtin_bswap64(x); }
+
+void field_path(char *out, size_t cap, const char *dir, const char *stem, int ti, int block) {
+  if (dir && dir[0])
+    snprintf(out, cap, "%s/%s%5.5d_%1.1d.dat", dir, stem, ti, block);
+  else
+    snprintf(out, cap, "%s%5.5d_%1.1d.dat", stem, ti, block);
+}
+
+}  // namespace
+}  // namespace cles
+
+using namespace cles;
+
+extern "C" {
+
+int cles_petsc_vec_write(const char *path, const double *x, long long n, int block_size) {
+  if (!path || (!x && n > 0) || n < 0 || n > INT32_MAX || block_size < 1)
+    return set_error(nullptr, CLES_EINVAL, "cles_petsc_vec_write: bad arguments");
+  FILE *f = fopen(path, "wb");
+  if (!f) return set_error(nullptr, CLES_EINVAL, "cannot create %s: %s", path, strerror(errno));
+  const uint32_t head[2] = {be32((uint32_t)kVecClassId), be32((uint32_t)n)};
+  bool ok = fwrite(head, sizeof(head), 1, f) == 1;
+  std::vector<uint64_t> buf(1 << 16);
+  for (long long o = 0; ok && o < n; o += (long long)buf.size()) {
+    const long long m = n - o < (long long)buf.size() ? n - o : (long long)buf.size();
+    for (long long q = 0; q < m; q++) {
+      uint64_t u;
+      memcpy(&u, x + o + q, 8);
+      buf[q] = be64(u);
+    }
+    ok = fwrite(buf.data(), 8, (size_t)m, f) == (size_t)m;
+  }
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) return set_error(nullptr, CLES_EINVAL, "write error on %s", path);
+  if (block_size > 1) {
+    std::string info = std::string(path) + ".info";
+    FILE *g = fopen(info.c_str(), "w");
+    if (!g) return set_error(nullptr, CLES_EINVAL, "cannot create %s", info.c_str());
+    fprintf(g, "-vecload_block_size %d\n", block_size);
+    if (fclose(g) != 0) return set_error(nullptr, CLES_EINVAL, "write error on %s", info.c_str());
+  }
+  return CLES_OK;
+}
+
+int cles_petsc_vec_read(const char *path, double *x, long long n) {
+  if (!path || (!x && n > 0) || n < 0) return set_error(nullptr, CLES_EINVAL, "cles_petsc_vec_read: bad arguments");
+  FILE *f = fopen(path, "rb");
+  if (!f) return set_error(nullptr, CLES_EINVAL, "cannot open %s: %s", path, strerror(errno));
+  uint32_t head[2];
+  if (fread(head, sizeof(head), 1, f) != 1) {
+    fclose(f);
+    return set_error(nullptr, CLES_EINVAL, "%s: truncated header", path);
+  }
+  const int32_t id = (int32_t)be32(head[0]), len = (int32_t)be32(head[1]);
+  if (id != kVecClassId) {
+    fclose(f);
+    return set_error(nullptr, CLES_EINVAL, "%s: not a PETSc binary Vec (class id %d)", path, id);
+  }
+  if ((long long)len != n) {
+    fclose(f);
+    return set_error(nullptr, CLES_EINVAL, "%s: holds %d values, expected %lld", path, len, n);
+  }
+  std::vector<uint64_t> buf(1 << 16);
+  bool ok = true;
+  for (long long o = 0; ok && o < n; o += (long long)buf.size()) {
+    const long long m = n - o < (long long)buf.size() ? n - o : (long long)buf.size();
+    ok = fread(buf.data(), 8, (size_t)m, f) == (size_t)m;
+    for (long long q = 0; ok && q < m; q++) {
+      const uint64_t u = be64(buf[q]);
+      memcpy(x + o + q, &u, 8);
+    }
+  }
+  fclose(f);
+  if (!ok) return set_error(nullptr, CLES_EINVAL, "%s: truncated data", path);
+  return CLES_OK;
+}
+
+int cles_write_fields(cles_ctx *c, const char *dir, int block) {
+  if (!c) return CLES_EINVAL;
+  if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
+  const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
+  std::vector<double> q((size_t)N * 5), p((size_t)N), nv((size_t)N);
+  int r = cles_get_state(c, q.data(), p.data(), CLES_LAYOUT_GLOBAL);
+  if (r) return r;
+  if ((r = cles_get_nvert(c, nv.data()))) return r;
+  char path[1024];
+  const int ti = c->ti;
+  // the order of Q_P_Binary_Output (main.c:246-300)
+  field_path(path, sizeof(path), dir, "Qfield", ti, block);
+  if ((r = cles_petsc_vec_write(path, q.data(), N * 5, 5))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  if (c->p.les) {
+    std::vector<double> sig((size_t)N * 3);
+    if ((r = cles_get_les(c, nullptr, nullptr, sig.data()))) return r;
+    field_path(path, sizeof(path), dir, "sigma", ti, block);
+    if ((r = cles_petsc_vec_write(path, sig.data(), N * 3, 3))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  }
+  field_path(path, sizeof(path), dir, "pfield", ti, block);
+  if ((r = cles_petsc_vec_write(path, p.data(), N, 1))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  field_path(path, sizeof(path), dir, "nvfield", ti, block);
+  if ((r = cles_petsc_vec_write(path, nv.data(), N, 1))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  if (c->p.les) {
+    if ((r = cles_get_les(c, nullptr, p.data(), nullptr))) return r;
+    field_path(path, sizeof(path), dir, "nu_t", ti, block);
+    if ((r = cles_petsc_vec_write(path, p.data(), N, 1))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  }
+  return CLES_OK;
+}
+
+int cles_read_fields(cles_ctx *c, const char *dir, int ti, int block) {
+  if (!c) return CLES_EINVAL;
+  const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
+  std::vector<double> q((size_t)N * 5), p((size_t)N), nv((size_t)N);
+  char path[1024];
+  int r;
+  // the order of Q_P_Binary_Input (main.c:215-243)
+  field_path(path, sizeof(path), dir, "pfield", ti, block);
+  if ((r = cles_petsc_vec_read(path, p.data(), N))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  field_path(path, sizeof(path), dir, "nvfield", ti, block);
+  if ((r = cles_petsc_vec_read(path, nv.data(), N))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  field_path(path, sizeof(path), dir, "Qfield", ti, block);
+  if ((r = cles_petsc_vec_read(path, q.data(), N * 5))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  // RestartApplication (main.c:627-650): Nvert, P and Q as read, Qold = Q
+  if ((r = cles_set_nvert(c, nv.data()))) return r;
+  if ((r = cles_set_state_qp(c, q.data(), p.data(), CLES_LAYOUT_GLOBAL))) return r;
+  c->ti = ti;
+  return CLES_OK;
+}
+
+}  // extern "C"

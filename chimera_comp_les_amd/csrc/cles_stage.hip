@@ -30,7 +30,7 @@ namespace cles {
 
 namespace {
 
-constexpr int NT = 256;    // threads per workgroup, one interior node each
+constexpr int NT = 256;    // threads per workgroup of the larger tile shapes
 // occupancy target of the stage kernels (waves per SIMD the register
 // allocation must allow)
 #ifndef CLES_STAGE_WAVES
@@ -40,10 +40,11 @@ constexpr int NSLOT = 64;  // spread of the inf-norm atomics
 
 // Tile of interior nodes owned by one workgroup, with the face counts of its
 // convective/viscous face buffer and its primitive window (one-node halo).
-template <int X, int Y, int Z>
+template <int X, int Y, int Z, int N = NT>
 struct Tile {
-  static_assert(X * Y * Z <= NT, "at most one node per thread");
-  static constexpr int TX = X, TY = Y, TZ = Z, NN = X * Y * Z;
+  static_assert(X * Y * Z <= N, "at most one node per thread");
+  static_assert(N % 64 == 0, "whole waves");
+  static constexpr int TX = X, TY = Y, TZ = Z, NN = X * Y * Z, NT = N;
   static constexpr int NFI = (X + 1) * Y * Z, NFJ = X * (Y + 1) * Z, NFK = X * Y * (Z + 1);
   static constexpr int NF = NFI + NFJ + NFK;
   static constexpr int WX = X + 2, WY = Y + 2, WZ = Z + 2, NW = WX * WY * WZ;
@@ -57,6 +58,10 @@ typedef Tile<64, 2, 2> TileD;
 // (6x6x6: 252 faces; 8x5x5: at most 240), at 216 and 200 nodes per workgroup
 typedef Tile<6, 6, 6> TileE;
 typedef Tile<8, 5, 5> TileF;
+// one- and two-wave workgroups: no wave waits at a barrier for another, at
+// the price of more halo work per node (4x4x3: at most 64 faces; 8x4x3: at most 128)
+typedef Tile<4, 4, 3, 64> TileG;
+typedef Tile<8, 4, 3, 128> TileH;
 
 struct StageArgs {
   Grid g;
@@ -736,7 +741,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   Win win{i0 - 1, j0 - 1, k0 - 1};
   if (VISC) {
     // stage u, v, w, p/rho of the tile and a one-node halo
-    for (int w = t; w < NW; w += NT) {
+    for (int w = t; w < NW; w += TL::NT) {
       int wx = w % WX, wy = (w / WX) % WY, wz = w / (WX * WY);
       int ii = win.i0 + wx, jj = win.j0 + wy, kk = win.k0 + wz;
       if (ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
@@ -764,7 +769,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     if (STG) {
       const int nb = bx * by * bz;
       const double *mt = metric(a, d);
-      for (int w = t; w < nb; w += NT) {
+      for (int w = t; w < nb; w += TL::NT) {
         const int ii = ox + w % bx, jj = oy + (w / bx) % by, kk = oz + w / (bx * by);
         if (ii < 0 || jj < 0 || kk < 0 || ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
         const long long m = nid(g, kk, jj, ii);
@@ -786,7 +791,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       __syncthreads();
     }
     // ---- convective fluxes of direction d -> LDS
-    for (int f = t; f < nfd; f += NT) {
+    for (int f = t; f < nfd; f += TL::NT) {
       const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
       const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
       if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
@@ -809,7 +814,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     __syncthreads();
     if (!VISC) continue;
     // ---- viscous fluxes of direction d -> LDS
-    for (int f = t; f < nfd; f += NT) {
+    for (int f = t; f < nfd; f += TL::NT) {
       const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
       const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
       if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
@@ -899,7 +904,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
 }
 
 template <class TL, typename T, bool LES, bool VISC, int MODE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CLES_STAGE_WAVES))) void stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(CLES_STAGE_WAVES))) void stage_kernel(StageArgs a) {
   // XCD-aware tile numbering: blockIdx % 8 selects the XCD, each XCD walks a
   // contiguous range of tiles and keeps their shared halos in its own L2.
   const int b = blockIdx.x;
@@ -910,7 +915,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CLES_STAGE_W
 // independent grids; tstart[r] is the first workgroup of grid r (prefix sum
 // of the tile counts, tstart[nreq] = total).
 template <class TL, typename T, bool LES, bool VISC, int MODE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CLES_STAGE_WAVES))) void stage_batch_kernel(const StageArgs *__restrict__ tab, const int *__restrict__ tstart,
+__global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(CLES_STAGE_WAVES))) void stage_batch_kernel(const StageArgs *__restrict__ tab, const int *__restrict__ tstart,
                                                          int nreq) {
   const int b = blockIdx.x;
   int lo = 0, hi = nreq;  // largest r with tstart[r] <= b
@@ -932,9 +937,9 @@ void tile_counts(const Grid &g, StageArgs &a) {
 template <class TL, typename T, bool LES, bool VISC>
 hipError_t launch_t(const StageArgs &a, int mode, int nblocks, hipStream_t s) {
   if (mode == 0)
-    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 0>), dim3(nblocks), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 0>), dim3(nblocks), dim3(TL::NT), 0, s, a);
   else
-    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 1>), dim3(nblocks), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 1>), dim3(nblocks), dim3(TL::NT), 0, s, a);
   return hipGetLastError();
 }
 
@@ -954,7 +959,7 @@ hipError_t launch_shape(const cles_params &p, StageArgs &a, int mode, hipStream_
 
 template <class TL, typename T, bool VISC>
 hipError_t launch_batch_t(const StageArgs *tab, const int *tstart, int nreq, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((stage_batch_kernel<TL, T, false, VISC, 1>), dim3(nblocks), dim3(NT), 0, s, tab, tstart, nreq);
+  hipLaunchKernelGGL((stage_batch_kernel<TL, T, false, VISC, 1>), dim3(nblocks), dim3(TL::NT), 0, s, tab, tstart, nreq);
   return hipGetLastError();
 }
 
@@ -1009,6 +1014,8 @@ int stage_batch_args(cles_ctx *c, int in, int out, int istage, const double coef
     case 3: tile_counts<TileD>(a.g, a); break;
     case 4: tile_counts<TileE>(a.g, a); break;
     case 5: tile_counts<TileF>(a.g, a); break;
+    case 6: tile_counts<TileG>(a.g, a); break;
+    case 7: tile_counts<TileH>(a.g, a); break;
     default: tile_counts<TileA>(a.g, a); break;
   }
   memcpy(dst, &a, sizeof(a));
@@ -1024,6 +1031,8 @@ hipError_t launch_stage_batch(int tile, int fp32, int visc, const void *tab, con
     case 3: return launch_batch_shape<TileD>(fp32, visc, t, tstart, nreq, nblocks, s);
     case 4: return launch_batch_shape<TileE>(fp32, visc, t, tstart, nreq, nblocks, s);
     case 5: return launch_batch_shape<TileF>(fp32, visc, t, tstart, nreq, nblocks, s);
+    case 6: return launch_batch_shape<TileG>(fp32, visc, t, tstart, nreq, nblocks, s);
+    case 7: return launch_batch_shape<TileH>(fp32, visc, t, tstart, nreq, nblocks, s);
     default: return launch_batch_shape<TileA>(fp32, visc, t, tstart, nreq, nblocks, s);
   }
 }
@@ -1038,6 +1047,8 @@ int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const doubl
     case 3: e = launch_shape<TileD>(c->p, a, mode, c->stream); break;
     case 4: e = launch_shape<TileE>(c->p, a, mode, c->stream); break;
     case 5: e = launch_shape<TileF>(c->p, a, mode, c->stream); break;
+    case 6: e = launch_shape<TileG>(c->p, a, mode, c->stream); break;
+    case 7: e = launch_shape<TileH>(c->p, a, mode, c->stream); break;
     default: e = launch_shape<TileA>(c->p, a, mode, c->stream); break;
   }
   CLES_HIP(c, e);

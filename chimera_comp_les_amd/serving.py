@@ -31,7 +31,7 @@ import chimera_comp_les_amd as cl
 from chimera_comp_les_amd import cases
 
 SIZES = (32, 48, 64, 96, 128)
-TILES = {0: "32x4x2", 1: "16x4x4", 2: "16x8x2", 3: "64x2x2", 4: "6x6x6", 5: "8x5x5"}
+TILES = {0: "32x4x2", 1: "16x4x4", 2: "16x8x2", 3: "64x2x2", 4: "6x6x6", 5: "8x5x5", 6: "4x4x3/64", 7: "8x4x3/128"}
 
 Request = collections.namedtuple("Request", "rid kind N Ma Re steps arrival")
 

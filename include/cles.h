@@ -134,8 +134,29 @@ int cles_ibm_get_list(cles_ctx *c, cles_ibm_info *out, int n);
 /* Upload the conserved state; sets Q, Qold and the pressure by EqOfState.
  * Replaces the tail of SetInitialCondition (rhs.c:424-433). */
 int cles_set_state(cles_ctx *c, const double *Q, int layout);
+/* Upload Q and P as given (no EqOfState); Q becomes Qold too.  The state
+ * RestartApplication (main.c:627-650) sets from the field files. */
+int cles_set_state_qp(cles_ctx *c, const double *Q, const double *P, int layout);
 /* Download Q (5 per node) and/or P (1 per node); either may be NULL. */
 int cles_get_state(cles_ctx *c, double *Q, double *P, int layout);
+
+/* Binary field files of the reference (PETSc binary Vec format, single-rank
+ * natural ordering, 32-bit PetscInt; see cles_io.hip).
+ * cles_write_fields: Q_P_Binary_Output (main.c:246-300) for the current time
+ * step: Qfield%05d_%1d.dat (+ .info), with LES sigma (+ .info), pfield,
+ * nvfield, with LES nu_t; dir may be NULL (current directory).
+ * cles_read_fields: Q_P_Binary_Input (main.c:215-243) + RestartApplication
+ * (main.c:627-650): reads pfield, nvfield, Qfield of step ti, sets Nvert, Q,
+ * Qold = Q, P as read and the time step to ti.  An immersed body is
+ * classified again afterwards (cles_ibm_search), as main.c:870-877 does. */
+int cles_write_fields(cles_ctx *c, const char *dir, int block);
+int cles_read_fields(cles_ctx *c, const char *dir, int ti, int block);
+/* One PETSc binary Vec file: class id 1211214 and length as big-endian
+ * int32, then n big-endian doubles; block_size > 1 also writes <path>.info
+ * ("-vecload_block_size"). Host only; no context or device needed. */
+int cles_petsc_vec_write(const char *path, const double *x, long long n, int block_size);
+/* Read one PETSc binary Vec file of exactly n values (CLES_EINVAL otherwise). */
+int cles_petsc_vec_read(const char *path, double *x, long long n);
 /* Metrics as formed by FormMetrics: csi, eta, zet (3 per node), aj. */
 int cles_get_metrics(cles_ctx *c, double *csi, double *eta, double *zet, double *aj);
 /* LES fields: Cs (CS), nu_t (Nu_t), sigma (Sigma, 3 per node); any may be NULL. */
@@ -202,8 +223,9 @@ int cles_batch_remove(cles_batch *b, cles_ctx *c);
 int cles_batch_size(const cles_batch *b);
 size_t cles_batch_used(const cles_batch *b);
 /* Stage-kernel tile shape of the batched launches: 0 32x4x2, 1 16x4x4,
- * 2 16x8x2, 3 64x2x2, 4 6x6x6, 5 8x5x5 (chosen by the autotuner of the
- * serving loop). */
+ * 2 16x8x2, 3 64x2x2, 4 6x6x6, 5 8x5x5 (256-thread workgroups), 6 4x4x3
+ * (64 threads), 7 8x4x3 (128 threads); chosen by the autotuner of the
+ * serving loop. */
 int cles_batch_set_tile(cles_batch *b, int tile);
 /* Advance every member by nsteps time steps (one graph replay per step) and
  * wait for them. */

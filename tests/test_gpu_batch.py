@@ -49,7 +49,7 @@ def test_footprint_is_the_allocation():
         s.close()
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_mix_equals_solo(tile):
     reqs = [request(*m) for m in MIX]
     ref = [solo(r, 4) for r in reqs]

@@ -4,6 +4,8 @@ Every case runs the oracle and libcles.so on the same seeded inputs at sizes
 the oracle finishes in well under a second.  Tolerances are the FP64 bar
 (1e-12) defined in parity_util.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -165,7 +167,7 @@ def test_steps_synthetic_inflow(les):
     assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_tile_shapes_agree(tile):
     """Every stage-kernel tile shape gives the bits of the default shape, on
     grids whose interiors leave partial tiles at the far edges (workgroups of
@@ -193,3 +195,28 @@ def test_tile_shapes_agree(tile):
         o.set_state(Q)
         o.step(3)
         assert state_err(Qg, o.state()[0]) <= STATE_TOL
+
+
+def test_field_files_restart(tmp_path):
+    """Q_P_Binary_Output after some steps, then Q_P_Binary_Input +
+    RestartApplication into a fresh context: the state, P as written, and the
+    continuation are bitwise those of the uninterrupted run (inviscid and
+    viscous without LES; with LES the reference does not save Cs and the
+    stale lUcat either)."""
+    for cfg, coor, Q in (small_vortex(N=12), channel_case(bc=(7, 7, 1, 1, 45, 8), iso=2)):
+        a = cl.Solver(cfg, coor)
+        a.set_state(Q)
+        a.advance(3)
+        a.write_fields(str(tmp_path), block=0)
+        assert os.path.exists(tmp_path / ("Qfield%05d_0.dat" % a.ti))
+        b = cl.Solver(cfg, coor)
+        b.read_fields(a.ti, str(tmp_path), block=0)
+        assert b.ti == a.ti
+        Qa, Pa = a.state()
+        Qb, Pb = b.state()
+        assert np.array_equal(Qa, Qb) and np.array_equal(Pa, Pb)
+        a.advance(2)
+        b.advance(2)
+        assert np.array_equal(a.state()[0], b.state()[0])
+        a.close()
+        b.close()
