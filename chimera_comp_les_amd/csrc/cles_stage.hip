@@ -36,6 +36,12 @@ constexpr int NT = 256;    // threads per workgroup of the larger tile shapes
 #ifndef CLES_STAGE_WAVES
 #define CLES_STAGE_WAVES 2
 #endif
+// convective and viscous flux of a face computed in one pass (one barrier
+// pair per direction instead of two): 0 never, 1 always, 2 all forms but the
+// FP64 LES form (measured faster there as two passes)
+#ifndef CLES_STAGE_MERGE
+#define CLES_STAGE_MERGE 2
+#endif
 constexpr int NSLOT = 64;  // spread of the inf-norm atomics
 
 // Tile of interior nodes owned by one workgroup, with the face counts of its
@@ -701,7 +707,12 @@ template <class TL> __device__ __forceinline__ int fk_of(int lx, int ly, int lz)
 template <class TL, typename T, bool LES, bool VISC, int MODE>
 __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) {
   constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, WX = TL::WX, WY = TL::WY, NW = TL::NW;
-  constexpr int FC = VISC ? (LES ? 7 : 5) : 5;
+  // face buffer: the convective flux (5) and, computed in the same pass, the
+  // viscous flux (4) and the LES stresses (3) of each face of one direction
+  constexpr bool MERGE = VISC && (CLES_STAGE_MERGE == 1 || (CLES_STAGE_MERGE == 2 && !(LES && sizeof(T) == 8)));
+  constexpr int FV = VISC ? (LES ? 7 : 4) : 0;
+  constexpr int FC = MERGE ? 5 + FV : (VISC ? (LES ? 7 : 5) : 5);
+  constexpr int VO = MERGE ? 5 : 0;  // first viscous slot
   constexpr int NFD = TL::NFI > TL::NFJ ? (TL::NFI > TL::NFK ? TL::NFI : TL::NFK) : (TL::NFJ > TL::NFK ? TL::NFJ : TL::NFK);
   // staged node box of direction d: the tile widened by two nodes on both
   // sides along d (the four-point stencils of its faces)
@@ -807,39 +818,55 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       else conv_face_plain<T>(a, sv, d, fk, fj, fi, F);
 #pragma unroll
       for (int q = 0; q < 5; q++) Fs[q][f] = F[q];
-    }
-    __syncthreads();
+      if (MERGE) {
+        // ---- viscous flux of the same face
+        T G[4], al[3] = {T(0), T(0), T(0)};
+        visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, G, al);
 #pragma unroll
-    for (int q = 0; q < 5; q++) conv[q] = d == 0 ? Fs[q][hi] - Fs[q][lo] : conv[q] + Fs[q][hi] - Fs[q][lo];
-    __syncthreads();
-    if (!VISC) continue;
-    // ---- viscous fluxes of direction d -> LDS
-    for (int f = t; f < nfd; f += TL::NT) {
-      const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
-      const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
-      if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
-      T G[4], al[3] = {T(0), T(0), T(0)};
-      visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, G, al);
+        for (int q = 0; q < 4; q++) Fs[VO + q][f] = G[q];
+        if (LES) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) Fs[q][f] = G[q];
-      if (LES) {
-#pragma unroll
-        for (int q = 0; q < 3; q++) Fs[4 + q][f] = al[q];
+          for (int q = 0; q < 3; q++) Fs[VO + 4 + q][f] = al[q];
+        }
       }
     }
     __syncthreads();
 #pragma unroll
+    for (int q = 0; q < 5; q++) conv[q] = d == 0 ? Fs[q][hi] - Fs[q][lo] : conv[q] + Fs[q][hi] - Fs[q][lo];
+    if (!VISC) {
+      __syncthreads();
+      continue;
+    }
+    if (!MERGE) {
+      __syncthreads();
+      // ---- viscous fluxes of direction d -> LDS
+      for (int f = t; f < nfd; f += TL::NT) {
+        const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
+        const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
+        if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
+        T G[4], al[3] = {T(0), T(0), T(0)};
+        visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, G, al);
+#pragma unroll
+        for (int q = 0; q < 4; q++) Fs[q][f] = G[q];
+        if (LES) {
+#pragma unroll
+          for (int q = 0; q < 3; q++) Fs[4 + q][f] = al[q];
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
     for (int q = 0; q < 4; q++)
-      visc[q + 1] = d == 0 ? Fs[q][hi] - Fs[q][lo] : visc[q + 1] + Fs[q][hi] - Fs[q][lo];
+      visc[q + 1] = d == 0 ? Fs[VO + q][hi] - Fs[VO + q][lo] : visc[q + 1] + Fs[VO + q][hi] - Fs[VO + q][lo];
     if (les_e) {
       // face densities (r_n + r_n-1)/2, (r_n+1 + r_n)/2 along d
       const long long st = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
       const double *r = a.q;
       const T rl = T((r[n] + r[n - st]) * 0.5), rh = T((r[n + st] + r[n]) * 0.5);
-      const T p0 = rh * T(Fs[4][hi]), p1 = rh * T(Fs[5][hi]), p2 = rh * T(Fs[6][hi]);
-      E0 = d == 0 ? p0 - rl * T(Fs[4][lo]) : E0 + p0 - rl * T(Fs[4][lo]);
-      E1 = d == 0 ? p1 - rl * T(Fs[5][lo]) : E1 + p1 - rl * T(Fs[5][lo]);
-      E2 = d == 0 ? p2 - rl * T(Fs[6][lo]) : E2 + p2 - rl * T(Fs[6][lo]);
+      const T p0 = rh * T(Fs[VO + 4][hi]), p1 = rh * T(Fs[VO + 5][hi]), p2 = rh * T(Fs[VO + 6][hi]);
+      E0 = d == 0 ? p0 - rl * T(Fs[VO + 4][lo]) : E0 + p0 - rl * T(Fs[VO + 4][lo]);
+      E1 = d == 0 ? p1 - rl * T(Fs[VO + 5][lo]) : E1 + p1 - rl * T(Fs[VO + 5][lo]);
+      E2 = d == 0 ? p2 - rl * T(Fs[VO + 6][lo]) : E2 + p2 - rl * T(Fs[VO + 6][lo]);
     }
     __syncthreads();
   }
