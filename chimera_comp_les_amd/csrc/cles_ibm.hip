@@ -701,19 +701,17 @@ __global__ __launch_bounds__(kChainThreads) void interp_chain_kernel(InterpArgs 
   const int chunk = (nib + kChainThreads - 1) / kChainThreads;
   const int m0 = t * chunk < nib ? t * chunk : nib;
   const int m1 = (t + 1) * chunk < nib ? (t + 1) * chunk : nib;
-  if (a.iso != 2) {
-    for (int m = m0; m < m1; m++) write_node(a, m, a.w[W_RHO * nib + m], a.w[W_PINT * nib + m]);
+  if (a.iso != 2) {  // no chain: nodes are independent
+    for (int m = t; m < nib; m += kChainThreads) write_node(a, m, a.w[W_RHO * nib + m], a.w[W_PINT * nib + m]);
     return;
   }
-  if (a.sweep == 0) {  // Pa is the node's own interpolated pressure
-    double last = 0;
-    for (int m = m0; m < m1; m++) {
+  if (a.sweep == 0) {  // Pa is the node's own interpolated pressure: no chain
+    for (int m = t; m < nib; m += kChainThreads) {
       double rhob, pb;
       iso2_node(a, m, a.w[W_PINT * nib + m], rhob, pb);
       write_node(a, m, rhob, pb);
-      last = pb;
+      if (m == nib - 1) *a.carry = pb;
     }
-    if (m1 == nib && m0 < m1) *a.carry = last;
     return;
   }
   // estimate: affine map of each node, Pa_next = alpha + beta * Pa

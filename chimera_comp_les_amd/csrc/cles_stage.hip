@@ -763,6 +763,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       Ws[2 * NW + w] = T(a.q[m + 3 * np] / r);
       Ws[3 * NW + w] = T(a.p[m] / r - a.t_ref);  // t_ref = 0 in FP64: only gradients of p/rho are used
     }
+    __syncthreads();  // the window is read by other threads' viscous faces
   }
 #pragma unroll
   for (int d = 0; d < 3; d++) {  // unrolled: d is a constant in the face routines

@@ -85,6 +85,18 @@ def test_rhs_viscous(les):
     assert e <= RHS_TOL, e
 
 
+@pytest.mark.parametrize("les", [0, 2])
+def test_rhs_viscous_many_workgroups(les):
+    """The viscous RHS on a grid of a few hundred workgroups per launch."""
+    cfg, coor, Q = cases.tgv_case(N=36)
+    cfg["les"] = les
+    Q = perturbed(Q, 12)
+    o, g = _pair(cfg, coor, Q)
+    e = rhs_err(g.form_rhs(0), o.form_rhs(0))
+    g.close()
+    assert e <= RHS_TOL, e
+
+
 def test_steps_vortex():
     cfg, coor, Q = small_vortex(N=12)
     o, g = _pair(cfg, coor, Q)
