@@ -79,6 +79,9 @@ struct StageArgs {
   int bc[6];
   int fluxorder, eno, central, iso, has_ib;
   double ren, Ma, Pr, gamma, Tref, ratio, dt;
+  // uniform parts of the viscosity, formed once on the host with the same
+  // operations: 1/ren, 110.4/Tref, (1/ren) (1 + 110.4/Tref)
+  double inv_ren, suth_b, suth_a;
   // reference state of the FP32-bulk form: the float arithmetic carries the
   // deviations from it, the reference parts of the fluxes stay in double
   double rho_ref, p_ref, e_ref, h_ref, t_ref;
@@ -604,15 +607,14 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
   T r33 = dwdc * csi2 + dwde * eta2 + dwdz * zet2;
   T ajc = T(0.5 * (a.aj[gA] + a.aj[gB]));
   const double rA = a.q[gA], rB = a.q[gB];
-  T nu = T(1. / a.ren);
-  if (LES) nu = T(1. / a.ren * 0.5 * (rA + a.q[gA + 1]));  // rho at [i] and [i+1] on every face
+  T nu = T(a.inv_ren);
+  if (LES) nu = T(a.inv_ren * 0.5 * (rA + a.q[gA + 1]));  // rho at [i] and [i+1] on every face
   if (a.suth) {
-    const double C_sutherland = 110.4;
     double Tr = a.gamma * a.Ma * a.Ma * (a.p[gA] + a.p[gB]) / (rA + rB);
     if (Tr < 0)
-      nu = T(1. / a.ren);
+      nu = T(a.inv_ren);
     else
-      nu = T(1. / a.ren * (1 + C_sutherland / a.Tref) / (Tr + C_sutherland / a.Tref) * pow15_cr(Tr));
+      nu = T(a.suth_a / (Tr + a.suth_b) * pow15_cr(Tr));  // 1/ren (1 + C/Tref) / (Tr + C/Tref) Tr^1.5
   }
   T div = T(2. / 3.) * (dudc * csi0 + dude * eta0 + dudz * zet0 + dvdc * csi1 + dvde * eta1 + dvdz * zet1 +
                         dwdc * csi2 + dwde * eta2 + dwdz * zet2);
@@ -1017,6 +1019,12 @@ void fill_stage_args(cles_ctx *c, int in, int out, int istage, const double coef
   a.fluxorder = p.fluxorder; a.eno = p.eno_order; a.central = p.central; a.iso = p.isothermal;
   a.has_ib = c->has_ib;
   a.ren = p.ren; a.Ma = p.Ma; a.Pr = p.Pr; a.gamma = p.gamma; a.Tref = p.Tref; a.ratio = p.ratio; a.dt = p.dt;
+  {
+    const double C_sutherland = 110.4;  // rhs.c:1410
+    a.inv_ren = 1. / p.ren;
+    a.suth_b = C_sutherland / p.Tref;
+    a.suth_a = 1. / p.ren * (1 + C_sutherland / p.Tref);
+  }
   const bool f32 = p.precision == CLES_PREC_FP32BULK;
   a.rho_ref = f32 ? c->ref_rho : 0.0;
   a.p_ref = f32 ? c->ref_p : 0.0;
