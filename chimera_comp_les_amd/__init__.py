@@ -50,6 +50,7 @@ class CParams(C.Structure):
         ("isothermal", C.c_int), ("calc_sigma", C.c_int), ("abc", C.c_int),
         ("twoD", C.c_int), ("testfilter_ik", C.c_int),
         ("tistart", C.c_int), ("rstart", C.c_int), ("precision", C.c_int),
+        ("averaging", C.c_int),
     ]
 
 
@@ -64,6 +65,7 @@ EXPORTS = [
     "cles_batch_used", "cles_batch_set_tile", "cles_batch_advance", "cles_batch_time_step",
     "cles_batch_last_error", "cles_set_tile", "cles_batch_launch", "cles_batch_wait",
     "cles_set_state_qp", "cles_write_fields", "cles_read_fields", "cles_petsc_vec_write", "cles_petsc_vec_read",
+    "cles_get_averages",
 ]
 
 IBM_INFO_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
@@ -144,6 +146,7 @@ def load_library(path=LIB_PATH):
     L.cles_ibm_set_velocity.argtypes = [vp, dp, dp]
     L.cles_ibm_get_list.argtypes = [vp, C.POINTER(CIbmInfo), C.c_int]
     L.cles_set_state_qp.argtypes = [vp, dp, dp, C.c_int]
+    L.cles_get_averages.argtypes = [vp, dp, dp, dp, dp]
     L.cles_write_fields.argtypes = [vp, C.c_char_p, C.c_int]
     L.cles_read_fields.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
     L.cles_petsc_vec_write.argtypes = [C.c_char_p, dp, C.c_longlong, C.c_int]
@@ -353,6 +356,15 @@ class Solver:
         if n:
             out[:] = np.frombuffer(arr, dtype=IBM_INFO_DTYPE, count=n)
         return out
+
+    def averages(self):
+        """Do_averaging sums: Q (5), P, u^2 (3), (uv, vw, wu) per node."""
+        q = np.empty(self.shape + (5,))
+        p = np.empty(self.shape)
+        u2 = np.empty(self.shape + (3,))
+        uc = np.empty(self.shape + (3,))
+        self._check(self.L.cles_get_averages(self.h, _ptr(q), _ptr(p), _ptr(u2), _ptr(uc)))
+        return q, p, u2, uc
 
     def set_tile(self, tile):
         """Stage-kernel tile shape: 0 32x4x2, 1 16x4x4, 2 16x8x2, 3 64x2x2, 4 6x6x6, 5 8x5x5

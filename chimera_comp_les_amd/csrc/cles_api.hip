@@ -206,6 +206,7 @@ size_t ctx_footprint(const cles_params *p) {
     planes += 1 + 3 + 3;  // tfil2, cs, cs_alt, nut, ua
     if (p->les != 1 && p->les != 10) planes += 7 + 2 + 3;  // strain rates, LM, MM, LM/27, MM/27, 1/aj
   }
+  if (p->averaging) planes += 12;  // Do_averaging sums
   const size_t xfer = (size_t)((p->mx + 4) * (long long)(p->my + 4) * (p->mz + 4)) * 5;
   return (planes * np + xfer) * sizeof(double);
 }
@@ -292,6 +293,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
       TRY(dalloc(c, &c->ivol, np));
     }
   }
+  if (p->averaging) TRY(dalloc(c, &c->avg, 12 * np));
   {
     void *dn = nullptr;
     TRY(hipMalloc(&dn, 128 * sizeof(unsigned long long)) == hipSuccess ? 0 : set_error(c, CLES_ENOMEM, "norm buffer"));
@@ -311,7 +313,7 @@ void cles_destroy(cles_ctx *c) {
   if (!c) return;
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->ua, c->qsnap, c->tfil2, c->lmq, c->mmq, c->ivol};
+                    c->fil2, c->ua, c->qsnap, c->tfil2, c->lmq, c->mmq, c->ivol, c->avg};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   ibm_free(c);
@@ -393,6 +395,18 @@ int cles_get_metrics(cles_ctx *c, double *csi, double *eta, double *zet, double 
   if (eta && (r = download(c, c->eta, eta, 3, CLES_LAYOUT_GLOBAL))) return r;
   if (zet && (r = download(c, c->zet, zet, 3, CLES_LAYOUT_GLOBAL))) return r;
   if (aj && (r = download(c, c->aj, aj, 1, CLES_LAYOUT_GLOBAL))) return r;
+  return CLES_OK;
+}
+
+int cles_get_averages(cles_ctx *c, double *q_sum, double *p_sum, double *u2_sum, double *ucross_sum) {
+  if (!c) return CLES_EINVAL;
+  if (!c->avg) return set_error(c, CLES_ESTATE, "averaging is off (cles_params.averaging = 0)");
+  const long long np = c->g.np;
+  int r;
+  if (q_sum && (r = download(c, c->avg, q_sum, 5, CLES_LAYOUT_GLOBAL))) return r;
+  if (p_sum && (r = download(c, c->avg + 5 * np, p_sum, 1, CLES_LAYOUT_GLOBAL))) return r;
+  if (u2_sum && (r = download(c, c->avg + 6 * np, u2_sum, 3, CLES_LAYOUT_GLOBAL))) return r;
+  if (ucross_sum && (r = download(c, c->avg + 9 * np, ucross_sum, 3, CLES_LAYOUT_GLOBAL))) return r;
   return CLES_OK;
 }
 
@@ -512,7 +526,10 @@ static int flow_step_async(cles_ctx *c) {
     int r = launch_les(c);
     if (r) return r;
   }
-  return runge_kutta_async(c);
+  int r = runge_kutta_async(c);
+  if (r) return r;
+  // Flow_Solver (solvers.c:52-53): statistics after the Runge-Kutta stages
+  return p.averaging ? launch_averaging(c) : CLES_OK;
 }
 
 int cles_flow_step(cles_ctx *c) {

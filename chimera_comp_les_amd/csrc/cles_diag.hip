@@ -72,7 +72,54 @@ __global__ __launch_bounds__(kRedThreads) void integrals_final_kernel(const doub
     for (int c = 0; c < 6; c++) out[c] = sh[c][0];
 }
 
+// Do_averaging, les.c:1664-1747 (period 1 without -fish): Q_sum += Q and
+// P_sum += P (VecAXPY with alpha 1: y + x), then the squares and cross
+// products of the Ucat of the last CalcUCat (u = rhoU / rho of that stage
+// input, rhs.c:1346-1350; zero before any CalcUCat) on every owned node.
+struct AvgArgs {
+  long long n, np;
+  const double *q, *p, *qu;  // state, pressure, CalcUCat input (or null)
+  double *s;                 // 12 planes
+};
+
+__global__ __launch_bounds__(256) void averaging_kernel(AvgArgs a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n) return;
+  const long long np = a.np;
+  double *s = a.s;
+#pragma unroll
+  for (int c = 0; c < 5; c++) s[c * np + t] = s[c * np + t] + a.q[c * np + t];
+  s[5 * np + t] = s[5 * np + t] + a.p[t];
+  double u = 0., v = 0., w = 0.;
+  if (a.qu) {
+    const double r = a.qu[t];
+    u = a.qu[np + t] / r;
+    v = a.qu[2 * np + t] / r;
+    w = a.qu[3 * np + t] / r;
+  }
+  s[6 * np + t] += u * u;
+  s[7 * np + t] += v * v;
+  s[8 * np + t] += w * w;
+  s[9 * np + t] += u * v;   // uv
+  s[10 * np + t] += v * w;  // vw
+  s[11 * np + t] += w * u;  // wu
+}
+
 }  // namespace
+
+int launch_averaging(cles_ctx *c) {
+  if (!c->avg) return set_error(c, CLES_ESTATE, "averaging buffers not allocated (-averaging 0)");
+  AvgArgs a;
+  a.n = (long long)c->g.mx * c->g.my * c->g.mz;
+  a.np = c->g.np;
+  a.q = c->Q[c->cur];
+  a.p = c->P[c->cur];
+  a.qu = c->ucat_src >= 0 ? c->Q[c->ucat_src] : nullptr;
+  a.s = c->avg;
+  hipLaunchKernelGGL(averaging_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, c->stream, a);
+  CLES_HIP(c, hipGetLastError());
+  return CLES_OK;
+}
 }  // namespace cles
 
 using namespace cles;
@@ -95,3 +142,4 @@ extern "C" int cles_integrals(cles_ctx *c, double out[6]) {
   if (e != hipSuccess) return set_error(c, CLES_EDEVICE, "cles_integrals: %s", hipGetErrorString(e));
   return CLES_OK;
 }
+

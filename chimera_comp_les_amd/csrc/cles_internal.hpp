@@ -188,6 +188,7 @@ struct cles_ctx {
   int immersed;             // a surface was classified: RK steps end with the interpolation
   cles_ibm_dev *ibm;
   double *qsnap;            // 6 planes: Q and P before FormBCS(0) of an immersed step
+  double *avg;              // -averaging: 12 planes, Q_sum 5, P_sum 1, u^2 3, (uv, vw, wu) 3
   // state: three conserved-variable buffers with their pressures
   double *Q[3], *P[3];
   int cur;       // buffer holding Q^n (= Qold)
@@ -222,6 +223,7 @@ int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const
 // bctype b is implemented on side q (0..5 = i-, i+, j-, j+, k-, k+)
 bool supported_bc(int side, int b);
 int launch_les(cles_ctx *c);
+int launch_averaging(cles_ctx *c);
 // batched stages of several periodic grids (serving)
 size_t stage_args_bytes();
 int stage_batch_args(cles_ctx *c, int in, int out, int istage, const double coef[3], int tile, void *dst);

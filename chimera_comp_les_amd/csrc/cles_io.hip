@@ -8,8 +8,9 @@
 // values as big-endian doubles in the natural ordering (i fastest, the dof of
 // a node interleaved).  VecView also writes `<file>.info` with
 // "-vecload_block_size <dof>" for dof > 1.  File names follow the reference:
-// Qfield%05d_%1d.dat, pfield..., nvfield..., and with LES sigma... and
-// nu_t...; the number is the time step, the digit the block.
+// Qfield%05d_%1d.dat, pfield..., nvfield..., with LES sigma... and nu_t...,
+// with -averaging (after step 0) Qfield_ave..., su2_Sq..., su_cross... and
+// psum...; the number is the time step, the digit the block.
 #include <cerrno>
 #include <cstdint>
 #include <cstdio>
@@ -126,6 +127,18 @@ int cles_write_fields(cles_ctx *c, const char *dir, int block) {
   if ((r = cles_petsc_vec_write(path, p.data(), N, 1))) return set_error(c, r, "%s", cles_last_error(nullptr));
   field_path(path, sizeof(path), dir, "nvfield", ti, block);
   if ((r = cles_petsc_vec_write(path, nv.data(), N, 1))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  if (c->p.averaging && ti != 0) {  // main.c:274-300
+    std::vector<double> u2((size_t)N * 3), uc((size_t)N * 3);
+    if ((r = cles_get_averages(c, q.data(), p.data(), u2.data(), uc.data()))) return r;
+    field_path(path, sizeof(path), dir, "Qfield_ave", ti, block);
+    if ((r = cles_petsc_vec_write(path, q.data(), N * 5, 5))) return set_error(c, r, "%s", cles_last_error(nullptr));
+    field_path(path, sizeof(path), dir, "su2_Sq", ti, block);
+    if ((r = cles_petsc_vec_write(path, u2.data(), N * 3, 3))) return set_error(c, r, "%s", cles_last_error(nullptr));
+    field_path(path, sizeof(path), dir, "su_cross", ti, block);
+    if ((r = cles_petsc_vec_write(path, uc.data(), N * 3, 3))) return set_error(c, r, "%s", cles_last_error(nullptr));
+    field_path(path, sizeof(path), dir, "psum", ti, block);
+    if ((r = cles_petsc_vec_write(path, p.data(), N, 1))) return set_error(c, r, "%s", cles_last_error(nullptr));
+  }
   if (c->p.les) {
     if ((r = cles_get_les(c, nullptr, p.data(), nullptr))) return r;
     field_path(path, sizeof(path), dir, "nu_t", ti, block);

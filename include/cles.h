@@ -77,6 +77,7 @@ typedef struct cles_params {
                             sees it incremented (main.c:890-891)            */
   int rstart;            /* restart flag                                      */
   int precision;         /* CLES_PREC_*                                       */
+  int averaging;         /* -averaging (main.c:779): Do_averaging after steps */
 } cles_params;
 
 typedef struct cles_ctx cles_ctx;
@@ -144,7 +145,8 @@ int cles_get_state(cles_ctx *c, double *Q, double *P, int layout);
  * natural ordering, 32-bit PetscInt; see cles_io.hip).
  * cles_write_fields: Q_P_Binary_Output (main.c:246-300) for the current time
  * step: Qfield%05d_%1d.dat (+ .info), with LES sigma (+ .info), pfield,
- * nvfield, with LES nu_t; dir may be NULL (current directory).
+ * nvfield, with -averaging (ti != 0) Qfield_ave, su2_Sq, su_cross, psum,
+ * with LES nu_t; dir may be NULL (current directory).
  * cles_read_fields: Q_P_Binary_Input (main.c:215-243) + RestartApplication
  * (main.c:627-650): reads pfield, nvfield, Qfield of step ti, sets Nvert, Q,
  * Qold = Q, P as read and the time step to ti.  An immersed body is
@@ -161,6 +163,12 @@ int cles_petsc_vec_read(const char *path, double *x, long long n);
 int cles_get_metrics(cles_ctx *c, double *csi, double *eta, double *zet, double *aj);
 /* LES fields: Cs (CS), nu_t (Nu_t), sigma (Sigma, 3 per node); any may be NULL. */
 int cles_get_les(cles_ctx *c, double *cs, double *nu_t, double *sigma);
+
+/* Running sums of Do_averaging (les.c:1664-1747), kept with -averaging and
+ * advanced after every cles_flow_step / cles_advance step: Q_sum (5 per
+ * node), P_sum, the squares (u^2, v^2, w^2) and cross products (uv, vw, wu)
+ * of the velocity of the last CalcUCat; any pointer may be NULL. */
+int cles_get_averages(cles_ctx *c, double *q_sum, double *p_sum, double *u2_sum, double *ucross_sum);
 
 /* EqOfState, rhs.c:4820: P from Q on every node. */
 int cles_eq_of_state(cles_ctx *c);

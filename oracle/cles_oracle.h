@@ -103,6 +103,10 @@ void orc_form_bcs(orc_ctx *c, double alfa);             /* rhs.c:477  */
 void orc_runge_kutta(orc_ctx *c);                       /* solvers.c:223 */
 void orc_les_update(orc_ctx *c);  /* les.c:135 + les.c:1477 */
 void orc_flow_step(orc_ctx *c);   /* main.c:900-974 one step */
+/* Do_averaging (les.c:1664-1747), called after a step as Flow_Solver does
+ * with -averaging; the sums: Q 5, P 1, u^2 3, (uv, vw, wu) 3 per node */
+void orc_do_averaging(orc_ctx *c);
+void orc_get_averages(const orc_ctx *c, double *q_sum, double *p_sum, double *u2_sum, double *ucross_sum);
 void orc_norms(const orc_ctx *c, double out[2]);       /* solvers.c:251-252 */
 
 /* Immersed body.  Classification + interpolation list

@@ -235,6 +235,7 @@ void orc_destroy(orc_ctx *c) {
   free(c->x_bp); free(c->y_bp); free(c->z_bp); free(c->nf_x); free(c->nf_y); free(c->nf_z);
   free(c->radvec); free(c->nv1); free(c->nv2); free(c->nv3); free(c->qvec); free(c->u_bp);
   free(c->uold_bp); free(c->ibl);
+  free(c->Ucat); free(c->Q_sum); free(c->P_sum); free(c->U2_sum); free(c->Ucross_sum);
   free(c);
 }
 

@@ -20,6 +20,11 @@ struct orc_ctx {
   orc_v3 *Sigma, *lSigma;
   double *CS, *lCS, *Nu_t, *lNu_t;
   orc_v3 *lUcat;
+  orc_v3 *Ucat;          /* global Ucat of the last CalcUCat (kept for Do_averaging) */
+  /* Do_averaging running sums (les.c:1664-1747), allocated on first use */
+  orc_q *Q_sum;
+  double *P_sum;
+  orc_v3 *U2_sum, *Ucross_sum;
   int ti;
   /* per-call temporaries the reference obtains with VecDuplicate */
   orc_q *fp1, *fp2, *fp3, *conv, *visc;

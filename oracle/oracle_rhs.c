@@ -424,7 +424,8 @@ void orc_form_convection_les(orc_ctx *c, int istage) {
 
 /* CalcUCat, rhs.c:1322-1365: Cartesian velocity on owned nodes, then G2L. */
 void orc_calc_ucat(orc_ctx *c) {
-  orc_v3 *ucat = malloc(c->nG * sizeof(orc_v3));
+  if (!c->Ucat) c->Ucat = malloc(c->nG * sizeof(orc_v3));
+  orc_v3 *ucat = c->Ucat;
   #pragma omp parallel for schedule(static)
   for (int k = 0; k < c->mz; k++)
     for (int j = 0; j < c->my; j++)
@@ -436,7 +437,45 @@ void orc_calc_ucat(orc_ctx *c) {
         u->z = q->rhoW / q->rho;
       }
   orc_g2l(c, (const double *)ucat, (double *)c->lUcat, 3);
-  free(ucat);
+}
+
+/* Do_averaging, les.c:1664-1747 (period 1 without -fish): Q_sum += Q and
+ * P_sum += P (VecAXPY with alpha 1), then over every owned node the squares
+ * and cross products of the global Ucat of the last CalcUCat. */
+void orc_do_averaging(orc_ctx *c) {
+  if (!c->Q_sum) {
+    c->Q_sum = calloc(c->nG, sizeof(orc_q));
+    c->P_sum = calloc(c->nG, sizeof(double));
+    c->U2_sum = calloc(c->nG, sizeof(orc_v3));
+    c->Ucross_sum = calloc(c->nG, sizeof(orc_v3));
+  }
+  if (!c->Ucat) c->Ucat = calloc(c->nG, sizeof(orc_v3)); /* no CalcUCat yet: Ucat is zero */
+  for (size_t n = 0; n < c->nG; n++) {
+    double *qs = (double *)&c->Q_sum[n];
+    const double *q = (const double *)&c->Q[n];
+    for (int v = 0; v < 5; v++) qs[v] = qs[v] + q[v];
+    c->P_sum[n] = c->P_sum[n] + c->P[n];
+  }
+  for (size_t n = 0; n < c->nG; n++) {
+    const orc_v3 u = c->Ucat[n];
+    c->U2_sum[n].x += u.x * u.x;
+    c->U2_sum[n].y += u.y * u.y;
+    c->U2_sum[n].z += u.z * u.z;
+    c->Ucross_sum[n].x += u.x * u.y; /* uv */
+    c->Ucross_sum[n].y += u.y * u.z; /* vw */
+    c->Ucross_sum[n].z += u.z * u.x; /* wu */
+  }
+}
+
+void orc_get_averages(const orc_ctx *c, double *q_sum, double *p_sum, double *u2_sum, double *ucross_sum) {
+  const size_t n = c->nG;
+  if (q_sum) { if (c->Q_sum) memcpy(q_sum, c->Q_sum, n * sizeof(orc_q)); else memset(q_sum, 0, n * sizeof(orc_q)); }
+  if (p_sum) { if (c->P_sum) memcpy(p_sum, c->P_sum, n * sizeof(double)); else memset(p_sum, 0, n * sizeof(double)); }
+  if (u2_sum) { if (c->U2_sum) memcpy(u2_sum, c->U2_sum, n * sizeof(orc_v3)); else memset(u2_sum, 0, n * sizeof(orc_v3)); }
+  if (ucross_sum) {
+    if (c->Ucross_sum) memcpy(ucross_sum, c->Ucross_sum, n * sizeof(orc_v3));
+    else memset(ucross_sum, 0, n * sizeof(orc_v3));
+  }
 }
 
 /* ---------------------------- formViscous ------------------------------- */

@@ -69,6 +69,8 @@ def lib():
         L.orc_runge_kutta.argtypes = [C.c_void_p]
         L.orc_les_update.argtypes = [C.c_void_p]
         L.orc_flow_step.argtypes = [C.c_void_p]
+        L.orc_do_averaging.argtypes = [C.c_void_p]
+        L.orc_get_averages.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.orc_norms.argtypes = [C.c_void_p, dp]
         L.orc_les_gradients.argtypes = [C.c_void_p, dp, dp]
         L.orc_les_gradients_literal.argtypes = [C.c_void_p, dp, dp]
@@ -204,9 +206,21 @@ class Oracle:
     def les_update(self):
         self.L.orc_les_update(self.h)
 
-    def step(self, n=1):
+    def step(self, n=1, averaging=False):
+        """n time steps (Flow_Solver); with averaging, Do_averaging after each."""
         for _ in range(n):
             self.L.orc_flow_step(self.h)
+            if averaging:
+                self.L.orc_do_averaging(self.h)
+
+    def averages(self):
+        """Do_averaging sums: Q (5), P, u^2 (3), (uv, vw, wu) per node."""
+        q = np.empty(self.shape + (5,))
+        p = np.empty(self.shape)
+        u2 = np.empty(self.shape + (3,))
+        uc = np.empty(self.shape + (3,))
+        self.L.orc_get_averages(self.h, _ptr(q), _ptr(p), _ptr(u2), _ptr(uc))
+        return q, p, u2, uc
 
     @property
     def ti(self):

@@ -232,3 +232,31 @@ def test_field_files_restart(tmp_path):
         assert np.array_equal(a.state()[0], b.state()[0])
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("case", ["tgv_les", "vortex"])
+def test_averaging_matches_oracle(case, tmp_path):
+    """-averaging: Do_averaging after every step (les.c:1664-1747), the sums
+    of Q and P and the squares / cross products of the last CalcUCat's
+    velocity (zero for the inviscid form, which never calls CalcUCat); the
+    averaging field files are written from step 1 on."""
+    if case == "vortex":
+        cfg, coor, Q = small_vortex(N=10)
+    else:
+        cfg, coor, Q = cases.tgv_case(N=10)
+        Q = perturbed(Q, 8, amp=1e-2)
+    cfg = dict(cfg)
+    cfg["averaging"] = 1
+    o, g = _pair(cfg, coor, Q)
+    o.step(3, averaging=True)
+    g.advance(3)
+    for a, b in zip(g.averages(), o.averages()):
+        assert state_err(a.reshape(a.shape[:3] + (-1,)), b.reshape(b.shape[:3] + (-1,))) <= STATE_TOL
+    u2 = g.averages()[2]
+    assert (np.abs(u2).max() == 0.0) == (case == "vortex")
+    g.write_fields(str(tmp_path))
+    for stem in ("Qfield_ave", "su2_Sq", "su_cross", "psum"):
+        assert os.path.exists(tmp_path / ("%s%05d_0.dat" % (stem, g.ti)))
+    q_sum = cl.petsc_vec_read(str(tmp_path / ("Qfield_ave%05d_0.dat" % g.ti)), Q.size)
+    assert np.array_equal(q_sum.reshape(Q.shape), g.averages()[0])
+    g.close()
