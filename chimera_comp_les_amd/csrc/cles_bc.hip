@@ -478,7 +478,10 @@ __global__ void eos_kernel(EosArgs a) {
 }  // namespace
 
 bool supported_bc(int side, int b) {
-  if (b == 7 || b == 0) return true;
+  // 1 (no-slip wall) on any side: FormBCS has no branch for it outside j
+  // (the nodes keep the stage update), the face fluxes treat it on every side
+  // (rhs.c:1658-1683, 1867-1889, 2065-2094, 2779-2787, 3088-3096)
+  if (b == 7 || b == 0 || b == 1) return true;
   if (side < 2) return b == 3 || b == 51;
   if (side < 4) return b == 3 || b == 10 || b == 1;
   if (side == 4) return b == 3 || b == 52 || b == 55 || b == 44 || b == 45 || b == 51 || b == 10 || b == 47;

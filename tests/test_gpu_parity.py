@@ -146,6 +146,8 @@ def test_norms_match():
     ((7, 7, 3, 3, 10, 49), 0, 0),
     ((3, 3, 10, 10, 3, 45), 0, 0),
     ((7, 7, 1, 1, 52, 45), 2, 2),
+    ((1, 1, 1, 1, 45, 8), 0, 2),    # no-slip walls on the i sides too
+    ((7, 7, 1, 1, 1, 1), 0, 0),     # and on the k sides
 ])
 def test_steps_channel_boundaries(bc, iso, les):
     cfg, coor, Q = channel_case(bc=bc, iso=iso, les=les)
