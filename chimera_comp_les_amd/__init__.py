@@ -65,7 +65,7 @@ EXPORTS = [
     "cles_batch_used", "cles_batch_set_tile", "cles_batch_advance", "cles_batch_time_step",
     "cles_batch_last_error", "cles_set_tile", "cles_batch_launch", "cles_batch_wait",
     "cles_set_state_qp", "cles_write_fields", "cles_read_fields", "cles_petsc_vec_write", "cles_petsc_vec_read",
-    "cles_get_averages",
+    "cles_get_averages", "cles_ibm_forces",
 ]
 
 IBM_INFO_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
@@ -147,6 +147,7 @@ def load_library(path=LIB_PATH):
     L.cles_ibm_get_list.argtypes = [vp, C.POINTER(CIbmInfo), C.c_int]
     L.cles_set_state_qp.argtypes = [vp, dp, dp, C.c_int]
     L.cles_get_averages.argtypes = [vp, dp, dp, dp, dp]
+    L.cles_ibm_forces.argtypes = [vp, dp, dp]
     L.cles_write_fields.argtypes = [vp, C.c_char_p, C.c_int]
     L.cles_read_fields.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
     L.cles_petsc_vec_write.argtypes = [C.c_char_p, dp, C.c_longlong, C.c_int]
@@ -355,6 +356,13 @@ class Solver:
         self._check(self.L.cles_ibm_get_list(self.h, arr, n))
         if n:
             out[:] = np.frombuffer(arr, dtype=IBM_INFO_DTYPE, count=n)
+        return out
+
+    def ibm_forces(self, center=(0.0, 0.0, 0.0)):
+        """Calc_forces_SI2: F(3), Cp(3), A(3), A_tot, M(3), Mdpdn(3), Ia(3)."""
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        out = np.empty(19)
+        self._check(self.L.cles_ibm_forces(self.h, _ptr(c), _ptr(out)))
         return out
 
     def averages(self):

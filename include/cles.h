@@ -131,6 +131,14 @@ int cles_ibm_search(cles_ctx *c, int n_v, const double *x_bp, const double *y_bp
 int cles_ibm_set_velocity(cles_ctx *c, const double *u, const double *uold);
 /* Copy the n interpolation nodes (n must equal the count) to out. */
 int cles_ibm_get_list(cles_ctx *c, cles_ibm_info *out, int n);
+/* Surface forces of the body, Calc_forces_SI2 (fsi.c:1312-2470) as
+ * Struc_Solver calls it at the start of a step (solvers.c:124): pressure at
+ * the interpolation nodes and the viscous stress of the last CalcUCat's
+ * velocity on their fluid-side faces, moments about center (fsi->x_c..z_c).
+ * out: F x,y,z; Cp x,y,z (pressure part); A x,y,z (projected areas); A_tot;
+ * M x,y,z; Mdpdn x,y,z; Ia x,y,z. */
+#define CLES_FORCE_N 19
+int cles_ibm_forces(cles_ctx *c, const double center[3], double out[CLES_FORCE_N]);
 
 /* Upload the conserved state; sets Q, Qold and the pressure by EqOfState.
  * Replaces the tail of SetInitialCondition (rhs.c:424-433). */

@@ -124,6 +124,10 @@ void orc_ibm_get_list(const orc_ctx *c, orc_ibm_info *out);
 /* surface velocities u, uold: 3 per vertex (zero after orc_ibm_search) */
 void orc_ibm_set_velocity(orc_ctx *c, const double *u, const double *uold);
 void orc_ibm_interpolation(orc_ctx *c, int add_dundt); /* ibm_iso.c:2572 */
+/* Calc_forces_SI2 (fsi.c:1312-2470) about center: F x,y,z; Cp x,y,z;
+ * A x,y,z; A_tot; M x,y,z; Mdpdn x,y,z; Ia x,y,z */
+#define ORC_FORCE_N 19
+void orc_ibm_forces(const orc_ctx *c, const double center[3], double out[ORC_FORCE_N]);
 /* geometry kernels, exported for known-answer tests */
 int orc_intsect_triangle(const double orig[3], const double dir[3], const double v0[3],
                          const double v1[3], const double v2[3], double *t, double *u, double *v);

@@ -70,6 +70,7 @@ def lib():
         L.orc_les_update.argtypes = [C.c_void_p]
         L.orc_flow_step.argtypes = [C.c_void_p]
         L.orc_do_averaging.argtypes = [C.c_void_p]
+        L.orc_ibm_forces.argtypes = [C.c_void_p, dp, dp]
         L.orc_get_averages.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.orc_norms.argtypes = [C.c_void_p, dp]
         L.orc_les_gradients.argtypes = [C.c_void_p, dp, dp]
@@ -212,6 +213,13 @@ class Oracle:
             self.L.orc_flow_step(self.h)
             if averaging:
                 self.L.orc_do_averaging(self.h)
+
+    def ibm_forces(self, center=(0.0, 0.0, 0.0)):
+        """Calc_forces_SI2: F(3), Cp(3), A(3), A_tot, M(3), Mdpdn(3), Ia(3)."""
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        out = np.empty(19)
+        self.L.orc_ibm_forces(self.h, _ptr(c), _ptr(out))
+        return out
 
     def averages(self):
         """Do_averaging sums: Q (5), P, u^2 (3), (uv, vw, wu) per node."""

@@ -78,3 +78,22 @@ def test_steps_sphere(iso, les):
     assert np.all(np.isfinite(Qc))
     assert state_err(Qg, Qc) <= STATE_TOL
     assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL
+
+
+@pytest.mark.parametrize("iso,les,steps", [(2, 0, 0), (2, 0, 3), (2, 2, 2), (1, 0, 2)])
+def test_surface_forces_sphere(iso, les, steps):
+    """Calc_forces_SI2 on the device: the per-node gather runs in parallel and
+    the sums follow the list order in one thread, so the nineteen outputs are
+    bitwise those of the oracle after the same steps (zero steps: no CalcUCat
+    yet, pressure only)."""
+    cfg, coor, Q, V, T = sphere(iso=iso, les=les)
+    o, g = compare_search(cfg, coor, Q, V, T)
+    if steps:
+        o.step(steps)
+        g.advance(steps)
+    center = (1.2, 1.2, 1.4)
+    fo = o.ibm_forces(center)
+    fg = g.ibm_forces(center)
+    g.close()
+    assert np.all(np.isfinite(fo)) and (steps == 0 or np.any(fo[0:3] != 0))
+    assert np.array_equal(fg, fo), np.c_[fg, fo, fg - fo]

@@ -329,3 +329,58 @@ def test_pow15_is_exactly_rounded():
         assert po.pow15(x) == exact, x
         assert abs(po.pow15(x) - libm.pow(x, 1.5)) <= np.spacing(exact)
     assert po.pow15(0.0) == 0.0
+
+
+# ---------------------------------------------------- surface forces (fsi.c)
+
+def _cartesian_sphere(p_of_z, n=20, L=2.0):
+    """Uniform Cartesian periodic box with an icosphere at its centre, fluid at
+    rest, pressure p(z); the interpolation list from the oracle's search."""
+    coor, h = cases.cartesian_grid(n, L=L)
+    z = coor[..., 2]
+    rho = np.ones(z.shape)
+    Q = cases.conserved(rho, 0 * z, 0 * z, 0 * z, p_of_z(z))
+    cfg = dict(mx=n + 2, my=n + 2, mz=n + 2, bctype=(7,) * 6, invicid=0, les=0, isothermal=2, ren=100.0,
+               Ma=0.3, dt=1e-3)
+    o = po.Oracle(cfg, coor)
+    o.set_state(Q)
+    V, T = cases.icosphere(2, radius=0.55, center=(0.5 * L, 0.5 * L, 0.5 * L))
+    assert o.ibm_search(V, T) > 0
+    return o, h
+
+
+def test_forces_of_uniform_pressure_vanish_exactly():
+    """A closed body in uniform pressure on a uniform grid: every fluid-side
+    face of the interpolation layer has a partner of the opposite sense with
+    the same area vector, so the pressure force is exactly zero (the fluid is
+    at rest and no CalcUCat has run: no viscous stress)."""
+    o, h = _cartesian_sphere(lambda z: 7.0 + 0 * z)
+    out = o.ibm_forces(center=(1.0, 1.0, 1.0))
+    assert np.all(out[0:6] == 0.0), out[0:6]
+    assert out[9] > 0                                   # A_tot: areas of the + faces
+    assert np.allclose(out[6:9], out[6:9][0], rtol=1e-12)  # equal projected areas of a sphere-like layer
+
+
+def test_forces_of_hydrostatic_pressure_count_the_body():
+    """p = z on a uniform grid: per (i, j) column the top interpolation node
+    (fluid above, + face) gives -p_top dx dy and the bottom one +p_bot dx dy,
+    so F_z = -dx dy dz * sum over columns of (nodes in the body's run - 1),
+    counted here from nvert alone (a discrete Archimedes law); F_x, F_y vanish
+    to round-off."""
+    o, h = _cartesian_sphere(lambda z: z)
+    out = o.ibm_forces(center=(1.0, 1.0, 1.0))
+    nv = o.nvert()
+    solid = nv >= 0.9
+    runs = 0
+    for jj in range(solid.shape[1]):
+        for ii in range(solid.shape[2]):
+            col = solid[:, jj, ii].astype(int)
+            if col.any():
+                edges = np.flatnonzero(np.diff(np.r_[0, col, 0]))
+                for a, b in zip(edges[::2], edges[1::2]):
+                    runs += (b - a) - 1
+    expect = -h ** 3 * runs
+    assert runs > 0 and abs(out[2] - expect) <= 1e-12 * abs(expect), (out[2], expect)
+    assert abs(out[0]) < 1e-12 and abs(out[1]) < 1e-12
+    # only the pressure part acts: the fluid is at rest
+    assert np.array_equal(out[0:3], out[3:6])
