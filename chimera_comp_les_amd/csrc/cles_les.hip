@@ -149,9 +149,14 @@ __device__ __forceinline__ long long seam_image(const Grid &g, int k, int j, int
 // velocity, then the six strain-rate components and |S| as the filter sees
 // them (periodic face copies of lSx..lSz, les.c:316-377; zero on other
 // boundary nodes; |S| has no periodic copy), with rho kept.  Every running
-// sum still adds its 27 terms in the reference's [k][j][i] order; the weights
-// 1/(|dk|+1) * 1/(|dj|+1) * 1/(|di|+1) are powers of two, exact as products.
+// sum still adds its 27 terms in the reference's [k][j][i] order.  The
+// weights 1/(|dk|+1) * 1/(|dj|+1) * 1/(|di|+1) are 1, 1/2, 1/4 or 1/8, so
+// w * x is exact and fma(w, x, s) rounds once, exactly as s + w * x does
+// (unless |w x| falls below 2^-1022, where the product itself would round);
+// their sum is 8 exactly, so vsum / wsum is vsum * 0.125 (the same exact
+// quotient rounded once).
 constexpr int FTX = 8, FTY = 8, FTZ = 4;
+constexpr double FILTER_INV_WSUM = 0.125;
 constexpr int FBX = FTX + 2, FBY = FTY + 2, FBZ = FTZ + 2, FNB = FBX * FBY * FBZ;
 static_assert(FTX * FTY * FTZ == 256, "one node per thread");
 
@@ -188,7 +193,6 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a, int ntx, int
   }
   __syncthreads();
   double s_rho = 0, s_ru = 0, s_rv = 0, s_rw = 0, s_ruu = 0, s_ruv = 0, s_ruw = 0, s_rvv = 0, s_rvw = 0, s_rww = 0;
-  double wsum = 0;
   if (act) {
 #pragma unroll 1
     for (int R = 0; R < 3; R++)
@@ -200,17 +204,16 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a, int ntx, int
           const double w = filter_w(R, Qd, Pd);
           const double rho = B[0][o], ru = B[1][o], rv = B[2][o], rw = B[3][o];
           const double u0 = B[4][o], u1 = B[5][o], u2 = B[6][o];
-          s_rho += w * rho;
-          s_ru += w * ru;
-          s_rv += w * rv;
-          s_rw += w * rw;
-          s_ruu += w * (ru * u0);
-          s_ruv += w * (ru * u1);
-          s_ruw += w * (ru * u2);
-          s_rvv += w * (rv * u1);
-          s_rvw += w * (rv * u2);
-          s_rww += w * (rw * u2);
-          wsum += w;
+          s_rho = fma(w, rho, s_rho);
+          s_ru = fma(w, ru, s_ru);
+          s_rv = fma(w, rv, s_rv);
+          s_rw = fma(w, rw, s_rw);
+          s_ruu = fma(w, ru * u0, s_ruu);
+          s_ruv = fma(w, ru * u1, s_ruv);
+          s_ruw = fma(w, ru * u2, s_ruw);
+          s_rvv = fma(w, rv * u1, s_rvv);
+          s_rvw = fma(w, rv * u2, s_rvw);
+          s_rww = fma(w, rw * u2, s_rww);
         }
   }
   __syncthreads();
@@ -252,31 +255,32 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a, int ntx, int
 #pragma unroll
         for (int c = 0; c < 6; c++) {
           const double Sv = B[1 + c][o];
-          sS[c] += w * Sv;
-          sSS[c] += w * (Sv * S * rho);
+          sS[c] = fma(w, Sv, sS[c]);
+          sSS[c] = fma(w, Sv * S * rho, sSS[c]);
         }
       }
   double Sx[6];
 #pragma unroll
   for (int c = 0; c < 6; c++) Sx[c] = B[1 + c][c0];
-  const double _rho = s_rho / wsum, _rhou = s_ru / wsum, _rhov = s_rv / wsum, _rhow = s_rw / wsum;
+  const double iw = FILTER_INV_WSUM;
+  const double _rho = s_rho * iw, _rhou = s_ru * iw, _rhov = s_rv * iw, _rhow = s_rw * iw;
   // full 3x3 tensors in row-major order from the six symmetric components
   const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
   double Lij[9];
-  Lij[0] = s_ruu / wsum - _rhou * _rhou / _rho;
-  Lij[1] = s_ruv / wsum - _rhou * _rhov / _rho;
-  Lij[2] = s_ruw / wsum - _rhou * _rhow / _rho;
-  Lij[3] = s_ruv / wsum - _rhov * _rhou / _rho;
-  Lij[4] = s_rvv / wsum - _rhov * _rhov / _rho;
-  Lij[5] = s_rvw / wsum - _rhov * _rhow / _rho;
-  Lij[6] = s_ruw / wsum - _rhow * _rhou / _rho;
-  Lij[7] = s_rvw / wsum - _rhow * _rhov / _rho;
-  Lij[8] = s_rww / wsum - _rhow * _rhow / _rho;
+  Lij[0] = s_ruu * iw - _rhou * _rhou / _rho;
+  Lij[1] = s_ruv * iw - _rhou * _rhov / _rho;
+  Lij[2] = s_ruw * iw - _rhou * _rhow / _rho;
+  Lij[3] = s_ruv * iw - _rhov * _rhou / _rho;
+  Lij[4] = s_rvv * iw - _rhov * _rhov / _rho;
+  Lij[5] = s_rvw * iw - _rhov * _rhow / _rho;
+  Lij[6] = s_ruw * iw - _rhow * _rhou / _rho;
+  Lij[7] = s_rvw * iw - _rhow * _rhov / _rho;
+  Lij[8] = s_rww * iw - _rhow * _rhow / _rho;
   double Sh6[6], SSh6[6];
 #pragma unroll
   for (int c = 0; c < 6; c++) {
-    Sh6[c] = sS[c] / wsum;
-    SSh6[c] = sSS[c] / wsum;
+    Sh6[c] = sS[c] * iw;
+    SSh6[c] = sSS[c] * iw;
   }
   double S_hat = 0;
 #pragma unroll
@@ -300,12 +304,61 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a, int ntx, int
   a.mmq[n0] = M_cat / 27.0;
 }
 
+// The 27-point means of les_cs_kernel and les_smooth_kernel run over tiles of
+// every node (boundary layers included): an 8x8x4 block of nodes and its
+// one-node halo staged in LDS, blocks placed contiguously per XCD.  Only
+// interior nodes read the halo, and their neighbours always lie in the grid.
+constexpr int MTX = 8, MTY = 8, MTZ = 4;
+constexpr int MBX = MTX + 2, MBY = MTY + 2, MBZ = MTZ + 2, MNB = MBX * MBY * MBZ;
+static_assert(MTX * MTY * MTZ == 256, "one node per thread");
+
+struct NodeTile {
+  int i0, j0, k0;  // first node of the block
+  int i, j, k;     // this thread's node
+  int c0;          // its LDS slot
+  bool valid;
+};
+__device__ __forceinline__ NodeTile node_tile(const Grid &g, int tile, int ntx, int nty) {
+  NodeTile T;
+  T.i0 = (tile % ntx) * MTX;
+  T.j0 = ((tile / ntx) % nty) * MTY;
+  T.k0 = (tile / (ntx * nty)) * MTZ;
+  const int t = threadIdx.x, lx = t % MTX, ly = (t / MTX) % MTY, lz = t / (MTX * MTY);
+  T.i = T.i0 + lx;
+  T.j = T.j0 + ly;
+  T.k = T.k0 + lz;
+  T.c0 = ((lz + 1) * MBY + (ly + 1)) * MBX + (lx + 1);
+  T.valid = T.i < g.mx && T.j < g.my && T.k < g.mz;
+  return T;
+}
+// node of halo slot w, or false when it lies outside the grid
+__device__ __forceinline__ bool halo_node(const Grid &g, const NodeTile &T, int w, int &kk, int &jj, int &ii) {
+  ii = T.i0 - 1 + w % MBX;
+  jj = T.j0 - 1 + (w / MBX) % MBY;
+  kk = T.k0 - 1 + w / (MBX * MBY);
+  return ii >= 0 && jj >= 0 && kk >= 0 && ii < g.mx && jj < g.my && kk < g.mz;
+}
+
 // <LM>/<MM> and clipping on every owned node (les.c:889-930, 1146-1161)
-__global__ __launch_bounds__(256) void les_cs_kernel(LesArgs a) {
+__global__ __launch_bounds__(256) void les_cs_kernel(LesArgs a, int ntx, int nty, int ntiles, int per_xcd) {
+  __shared__ double B[2][MNB];
   const Grid &g = a.g;
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (long long)g.mx * g.my * g.mz) return;
-  const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
+  const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (tile >= ntiles) return;
+  const NodeTile T = node_tile(g, tile, ntx, nty);
+  if (a.les != 3) {
+    for (int w = threadIdx.x; w < MNB; w += 256) {
+      int kk, jj, ii;
+      if (!halo_node(g, T, w, kk, jj, ii)) continue;
+      const long long m = nid(g, kk, jj, ii);
+      B[0][w] = a.lmq[m];  // lm[m] / 27.0 (zero on boundary nodes, never written)
+      B[1][w] = a.mmq[m];
+    }
+    __syncthreads();
+  }
+  if (!T.valid) return;
+  const int i = T.i, j = T.j, k = T.k;
+  const long long n = nid(g, k, j, i);
   const double nv = nvat(a, n);
   double cs = a.cs[n];
   if (interior(g, k, j, i)) {
@@ -317,12 +370,15 @@ __global__ __launch_bounds__(256) void les_cs_kernel(LesArgs a) {
         LM_avg = a.lm[n];
         MM_avg = a.mm[n];
       } else {
+#pragma unroll
         for (int c = -1; c <= 1; c++)
+#pragma unroll
           for (int b = -1; b <= 1; b++)
+#pragma unroll
             for (int aa = -1; aa <= 1; aa++) {
-              const long long m = nid(g, k + c, j + b, i + aa);
-              LM_avg += a.lmq[m];  // lm[m] / 27.0 (zero on boundary nodes, never written)
-              MM_avg += a.mmq[m];
+              const int o = T.c0 + (c * MBY + b) * MBX + aa;
+              LM_avg += B[0][o];
+              MM_avg += B[1][o];
             }
       }
       const double C = LM_avg / (MM_avg + LES_EPS);
@@ -347,21 +403,35 @@ __device__ __forceinline__ double lcs_at(const LesArgs &a, int k, int j, int i) 
 }
 
 // volume-weighted smoothing of Cs over 27 nodes (les.c:1258-1337)
-__global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs a) {
+__global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs a, int ntx, int nty, int ntiles, int per_xcd) {
+  __shared__ double B[2][MNB];
   const Grid &g = a.g;
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (long long)g.mx * g.my * g.mz) return;
-  const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
-  if (!interior(g, k, j, i) || !(nvat(a, n) < 0.1) || a.les == 3) {
+  const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (tile >= ntiles) return;
+  const NodeTile T = node_tile(g, tile, ntx, nty);
+  for (int w = threadIdx.x; w < MNB; w += 256) {
+    int kk, jj, ii;
+    if (!halo_node(g, T, w, kk, jj, ii)) continue;
+    B[0][w] = a.ivol[nid(g, kk, jj, ii)];  // 1. / aj
+    B[1][w] = lcs_at(a, kk, jj, ii);
+  }
+  __syncthreads();
+  if (!T.valid) return;
+  const long long n = nid(g, T.k, T.j, T.i);
+  if (!interior(g, T.k, T.j, T.i) || !(nvat(a, n) < 0.1) || a.les == 3) {
     a.cs_out[n] = a.cs[n];
     return;
   }
   double vs = 0.0, ws = 0.0;
+#pragma unroll
   for (int c = -1; c <= 1; c++)
+#pragma unroll
     for (int b = -1; b <= 1; b++)
+#pragma unroll
       for (int aa = -1; aa <= 1; aa++) {
-        const double w = a.ivol[nid(g, k + c, j + b, i + aa)];  // 1. / aj
-        vs += w * lcs_at(a, k + c, j + b, i + aa);
+        const int o = T.c0 + (c * MBY + b) * MBX + aa;
+        const double w = B[0][o];
+        vs += w * B[1][o];
         ws += w;
       }
   a.cs_out[n] = vs / ws;
@@ -445,10 +515,15 @@ int launch_les(cles_ctx *c) {
       hipLaunchKernelGGL(les_filter_kernel, dim3(8 * per_xcd), dim3(256), 0, c->stream, a, ntx, nty, ntiles, per_xcd);
     }
     CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(les_cs_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
-    CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(les_smooth_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
-    CLES_HIP(c, hipGetLastError());
+    {
+      const int ntx = (c->g.mx + MTX - 1) / MTX, nty = (c->g.my + MTY - 1) / MTY, ntz = (c->g.mz + MTZ - 1) / MTZ;
+      const int ntiles = ntx * nty * ntz, per_xcd = (ntiles + 7) / 8;
+      hipLaunchKernelGGL(les_cs_kernel, dim3(8 * per_xcd), dim3(256), 0, c->stream, a, ntx, nty, ntiles, per_xcd);
+      CLES_HIP(c, hipGetLastError());
+      hipLaunchKernelGGL(les_smooth_kernel, dim3(8 * per_xcd), dim3(256), 0, c->stream, a, ntx, nty, ntiles,
+                         per_xcd);
+      CLES_HIP(c, hipGetLastError());
+    }
     double *t = c->cs;
     c->cs = c->cs_alt;
     c->cs_alt = t;
