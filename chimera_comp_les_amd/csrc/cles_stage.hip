@@ -42,6 +42,26 @@ constexpr int NT = 256;    // threads per workgroup of the larger tile shapes
 #ifndef CLES_STAGE_MERGE
 #define CLES_STAGE_MERGE 2
 #endif
+// issue the viscous face loads of a direction early (forms with two passes
+// per direction): 0 no, 1 at the top of the direction, 2 after its
+// convective faces; PF_PART selects the loads: 1 metrics and aj, 2 the
+// others (rho, p, nu_t, filter width), 3 all (1 and 3 spill registers)
+#ifndef CLES_STAGE_PREFETCH
+#define CLES_STAGE_PREFETCH 2
+#endif
+#ifndef CLES_STAGE_PF_PART
+#define CLES_STAGE_PF_PART 2
+#endif
+// staging loops issue the loads of all their iterations before the LDS
+// stores: bit 0 the direction's node data, bit 1 the primitive window
+#ifndef CLES_STAGE_BATCH
+#define CLES_STAGE_BATCH 2
+#endif
+// load the node's update inputs (aj, state, old state) while the last
+// direction's viscous faces are evaluated
+#ifndef CLES_STAGE_PF_END
+#define CLES_STAGE_PF_END 0
+#endif
 constexpr int NSLOT = 64;  // spread of the inf-norm atomics
 
 // Tile of interior nodes owned by one workgroup, with the face counts of its
@@ -546,14 +566,54 @@ __device__ __forceinline__ void cross_deriv(const StageArgs &a, const T *W, int 
   }
 }
 
-// formViscous face (rhs.c:1487-1707, 1719-1918, 1926-2121); G[0..3] =
-// rhoU..rhoE flux, al = LES stress (alpha) of the face.
-template <class TL, typename T, bool LES>
-__device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d, int k, int j, int i, T G[4],
-                          T al[3]) {
-  constexpr int WX = TL::WX, WY = TL::WY, NW = TL::NW;
+// Global-memory inputs of one viscous face (its lower node A, upper node B):
+// the nine metric components, aj, rho and p at both nodes, rho at A + 1 and
+// (LES) nu_t at both nodes and the filter width at A.  Loaded in one place so
+// that a caller can issue the loads early and keep them in flight across the
+// phases before the face is evaluated.
+struct VisIn {
+  double m[9][2];  // csi0..2, eta0..2, zet0..2
+  double aj[2], r[2], p[2], nut[2], rA1, f2;
+};
+// part 1: the metrics and aj; part 2: the rest; 3: both
+template <bool LES, int PART = 3>
+__device__ __forceinline__ void visc_load(const StageArgs &a, int d, int k, int j, int i, VisIn &v) {
   const Grid &g = a.g;
   const long long np = g.np;
+  const long long gA = nid(g, k, j, i), gB = gA + (d == 0 ? 1 : d == 1 ? g.sy : g.sz);
+  const double *mt[3] = {a.csi, a.eta, a.zet};
+  if (PART & 1) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      v.m[c][0] = mt[c / 3][gA + (c % 3) * np];
+      v.m[c][1] = mt[c / 3][gB + (c % 3) * np];
+    }
+    v.aj[0] = a.aj[gA];
+    v.aj[1] = a.aj[gB];
+  }
+  if (!(PART & 2)) return;
+  v.r[0] = a.q[gA];
+  v.r[1] = a.q[gB];
+  if (a.suth) {
+    v.p[0] = a.p[gA];
+    v.p[1] = a.p[gB];
+  }
+  if (LES) {
+    v.rA1 = a.q[gA + 1];
+    v.nut[0] = a.nut[gA];
+    v.nut[1] = a.nut[gB];
+    v.f2 = a.fil2[gA];
+  }
+}
+
+// formViscous face (rhs.c:1487-1707, 1719-1918, 1926-2121); G[0..3] =
+// rhoU..rhoE flux, al = LES stress (alpha) of the face; v holds the face's
+// global-memory inputs (visc_load).
+template <class TL, typename T, bool LES>
+__device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d, int k, int j, int i, const VisIn &v,
+                          T G[4], T al[3]) {
+  constexpr int WX = TL::WX, WY = TL::WY, NW = TL::NW;
+  const Grid &g = a.g;
   const long long gs[3] = {1, g.sy, g.sz};
   const int ws[3] = {1, WX, WX * WY};
   const long long gA = nid(g, k, j, i), gB = gA + gs[d];
@@ -569,13 +629,12 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
     if (t == d) continue;
     cross_deriv<TL>(a, W, wA, wB, ws[t], gA, gB, gs[t], d == 0 && t == 2, dd[t], dT[t]);
   }
-  const double *cs = a.csi, *et = a.eta, *ze = a.zet;
-  T csi0 = T(0.5 * (cs[gA] + cs[gB])), csi1 = T(0.5 * (cs[gA + np] + cs[gB + np])),
-    csi2 = T(0.5 * (cs[gA + 2 * np] + cs[gB + 2 * np]));
-  T eta0 = T(0.5 * (et[gA] + et[gB])), eta1 = T(0.5 * (et[gA + np] + et[gB + np])),
-    eta2 = T(0.5 * (et[gA + 2 * np] + et[gB + 2 * np]));
-  T zet0 = T(0.5 * (ze[gA] + ze[gB])), zet1 = T(0.5 * (ze[gA + np] + ze[gB + np])),
-    zet2 = T(0.5 * (ze[gA + 2 * np] + ze[gB + 2 * np]));
+  T csi0 = T(0.5 * (v.m[0][0] + v.m[0][1])), csi1 = T(0.5 * (v.m[1][0] + v.m[1][1])),
+    csi2 = T(0.5 * (v.m[2][0] + v.m[2][1]));
+  T eta0 = T(0.5 * (v.m[3][0] + v.m[3][1])), eta1 = T(0.5 * (v.m[4][0] + v.m[4][1])),
+    eta2 = T(0.5 * (v.m[5][0] + v.m[5][1]));
+  T zet0 = T(0.5 * (v.m[6][0] + v.m[6][1])), zet1 = T(0.5 * (v.m[7][0] + v.m[7][1])),
+    zet2 = T(0.5 * (v.m[8][0] + v.m[8][1]));
   T m0, m1, m2, g11, g21, g31;
   if (d == 0) {
     m0 = csi0; m1 = csi1; m2 = csi2;
@@ -605,12 +664,12 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
   T r13 = dudc * csi2 + dude * eta2 + dudz * zet2;
   T r23 = dvdc * csi2 + dvde * eta2 + dvdz * zet2;
   T r33 = dwdc * csi2 + dwde * eta2 + dwdz * zet2;
-  T ajc = T(0.5 * (a.aj[gA] + a.aj[gB]));
-  const double rA = a.q[gA], rB = a.q[gB];
+  T ajc = T(0.5 * (v.aj[0] + v.aj[1]));
+  const double rA = v.r[0], rB = v.r[1];
   T nu = T(a.inv_ren);
-  if (LES) nu = T(a.inv_ren * 0.5 * (rA + a.q[gA + 1]));  // rho at [i] and [i+1] on every face
+  if (LES) nu = T(a.inv_ren * 0.5 * (rA + v.rA1));  // rho at [i] and [i+1] on every face
   if (a.suth) {
-    double Tr = a.gamma * a.Ma * a.Ma * (a.p[gA] + a.p[gB]) / (rA + rB);
+    double Tr = a.gamma * a.Ma * a.Ma * (v.p[0] + v.p[1]) / (rA + rB);
     if (Tr < 0)
       nu = T(a.inv_ren);
     else
@@ -647,7 +706,7 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
     const T CI = T(0.06);
     T sA = T(sigma_at(a, d, k, j, i));
     T sB = T(sigma_at(a, d, k + (d == 2), j + (d == 1), i + (d == 0)));
-    T nu_t = T(0.5) * (T(a.nut[gA]) * sA + T(a.nut[gB]) * sB);
+    T nu_t = T(0.5) * (T(v.nut[0]) * sA + T(v.nut[1]) * sB);
     T rho_c = T(0.5 * (rA + rB));
     T sigma_c = T(0.5) * (sA + sB);
     if ((a.bc[2 * d] == 1 && id == 0) || (a.bc[2 * d + 1] == 1 && id == m - 2)) nu_t = T(0);
@@ -656,7 +715,7 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
     T tz = (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * m0 + r23 * m1 + r33 * m2 - div * m2) * ajc * (nu_t * rho_c);
     T S2 = T(.5) * ((r11 - div) * (r11 - div) + r12 * r12 + r13 * r13 + r21 * r21 + (r22 - div) * (r22 - div) +
                     r23 * r23 + r31 * r31 + r32 * r32 + (r33 - div) * (r33 - div));
-    T f2 = T(a.fil2[gA]);  // (pow(1/aj, 1/3))^2 at the face's lower node
+    T f2 = T(v.f2);  // (pow(1/aj, 1/3))^2 at the face's lower node
     T Ssx = -S2 * f2 * CI * T(2.0) / T(3.0) * m0 * rho_c * sigma_c;
     T Ssy = -S2 * f2 * CI * T(2.0) / T(3.0) * m1 * rho_c * sigma_c;
     T Ssz = -S2 * f2 * CI * T(2.0) / T(3.0) * m2 * rho_c * sigma_c;
@@ -752,18 +811,58 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   const bool les_e = LES && VISC && valid && gj > 2 && gj < g.my - 2 && !(a.has_ib && !(a.nv[n] < 0.5));
   T E0 = T(0), E1 = T(0), E2 = T(0);
   Win win{i0 - 1, j0 - 1, k0 - 1};
+  // the node's update inputs, loaded early (CLES_STAGE_PF_END) or at the end
+  double ajn = 0., qin[5] = {0., 0., 0., 0., 0.}, qol[5] = {0., 0., 0., 0., 0.};
+  auto load_end = [&]() {
+    if (!valid) return;
+    ajn = a.aj[n];
+#pragma unroll
+    for (int q = 0; q < 5; q++) qin[q] = a.q[n + q * np];
+    if (MODE != 0) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) qol[q] = a.qold[n + q * np];
+    }
+  };
   if (VISC) {
     // stage u, v, w, p/rho of the tile and a one-node halo
-    for (int w = t; w < NW; w += TL::NT) {
-      int wx = w % WX, wy = (w / WX) % WY, wz = w / (WX * WY);
-      int ii = win.i0 + wx, jj = win.j0 + wy, kk = win.k0 + wz;
-      if (ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
-      long long m = nid(g, kk, jj, ii);
-      double r = a.q[m];
-      Ws[w] = T(a.q[m + np] / r);
-      Ws[NW + w] = T(a.q[m + 2 * np] / r);
-      Ws[2 * NW + w] = T(a.q[m + 3 * np] / r);
-      Ws[3 * NW + w] = T(a.p[m] / r - a.t_ref);  // t_ref = 0 in FP64: only gradients of p/rho are used
+    if (CLES_STAGE_BATCH & 2) {
+      constexpr int NIT = (NW + TL::NT - 1) / TL::NT;
+      double qv[NIT][5];
+      bool ok[NIT];
+#pragma unroll
+      for (int it = 0; it < NIT; it++) {
+        const int w = t + it * TL::NT;
+        const int ii = win.i0 + w % WX, jj = win.j0 + (w / WX) % WY, kk = win.k0 + w / (WX * WY);
+        ok[it] = w < NW && ii <= g.mx - 1 && jj <= g.my - 1 && kk <= g.mz - 1;
+        if (ok[it]) {
+          const long long m = nid(g, kk, jj, ii);
+#pragma unroll
+          for (int c = 0; c < 4; c++) qv[it][c] = a.q[m + c * np];
+          qv[it][4] = a.p[m];
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < NIT; it++) {
+        const int w = t + it * TL::NT;
+        if (!ok[it]) continue;
+        const double r = qv[it][0];
+        Ws[w] = T(qv[it][1] / r);
+        Ws[NW + w] = T(qv[it][2] / r);
+        Ws[2 * NW + w] = T(qv[it][3] / r);
+        Ws[3 * NW + w] = T(qv[it][4] / r - a.t_ref);
+      }
+    } else {
+      for (int w = t; w < NW; w += TL::NT) {
+        int wx = w % WX, wy = (w / WX) % WY, wz = w / (WX * WY);
+        int ii = win.i0 + wx, jj = win.j0 + wy, kk = win.k0 + wz;
+        if (ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
+        long long m = nid(g, kk, jj, ii);
+        double r = a.q[m];
+        Ws[w] = T(a.q[m + np] / r);
+        Ws[NW + w] = T(a.q[m + 2 * np] / r);
+        Ws[2 * NW + w] = T(a.q[m + 3 * np] / r);
+        Ws[3 * NW + w] = T(a.p[m] / r - a.t_ref);  // t_ref = 0 in FP64: only gradients of p/rho are used
+      }
     }
     __syncthreads();  // the window is read by other threads' viscous faces
   }
@@ -774,13 +873,66 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     // faces of this node: lo at the node, hi one step along d
     const int lo = (lz * ey + ly) * ex + lx;
     const int hi = lo + (d == 0 ? 1 : d == 1 ? ex : ex * ey);
+    // Two-pass forms whose faces of every direction fit one pass of the
+    // workgroup: the viscous face of this thread (f = t) loads its inputs
+    // before the convective sums, and the loads stay in flight through them
+    // (barriers wait for LDS traffic only).
+    constexpr bool PF = VISC && !MERGE && CLES_STAGE_PREFETCH && TL::NFI <= TL::NT && TL::NFJ <= TL::NT &&
+                        TL::NFK <= TL::NT;
+    VisIn vpre;
+    auto prefetch = [&]() {
+      if (PF && t < nfd) {
+        const int fx = t % ex, fy = (t / ex) % ey, fz = t / (ex * ey);
+        const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
+        if (fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2) visc_load<LES, CLES_STAGE_PF_PART>(a, d, fk, fj, fi, vpre);
+      }
+    };
+    if (CLES_STAGE_PREFETCH == 1) prefetch();
     // ---- node data of direction d -> LDS: the state, p and (LES) the
     // contravariant velocity, each computed once per node instead of once per
     // stencil it belongs to
     const int bx = TX + 4 * (d == 0), by = TY + 4 * (d == 1), bz = TZ + 4 * (d == 2);
     const int ox = i0 - 2 * (d == 0), oy = j0 - 2 * (d == 1), oz = k0 - 2 * (d == 2);
     constexpr int ns = NSM;
-    if (STG) {
+    if (STG && (CLES_STAGE_BATCH & 1)) {
+      const int nb = bx * by * bz;
+      const double *mt = metric(a, d);
+      constexpr int NIT = (NSM + TL::NT - 1) / TL::NT;
+      double sv9[NIT][9];
+      bool ok[NIT];
+#pragma unroll
+      for (int it = 0; it < NIT; it++) {
+        const int w = t + it * TL::NT;
+        const int ii = ox + w % bx, jj = oy + (w / bx) % by, kk = oz + w / (bx * by);
+        ok[it] = w < nb && ii >= 0 && jj >= 0 && kk >= 0 && ii <= g.mx - 1 && jj <= g.my - 1 && kk <= g.mz - 1;
+        if (ok[it]) {
+          const long long m = nid(g, kk, jj, ii);
+#pragma unroll
+          for (int c = 0; c < 5; c++) sv9[it][c] = a.q[m + c * np];
+          sv9[it][5] = a.p[m];
+          if (LES) {
+            sv9[it][6] = mt[m];
+            sv9[it][7] = mt[m + np];
+            sv9[it][8] = mt[m + 2 * np];
+          }
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < NIT; it++) {
+        const int w = t + it * TL::NT;
+        if (!ok[it]) continue;
+        const double rd = sv9[it][0], ru = sv9[it][1], rv = sv9[it][2], rw = sv9[it][3];
+#pragma unroll
+        for (int c = 0; c < 6; c++) Ss[c * ns + w] = sv9[it][c];
+        if (LES) {
+          if constexpr (sizeof(T) == 4)
+            Ss[6 * ns + w] = (sv9[it][6] * ru + sv9[it][7] * rv + sv9[it][8] * rw) * inv_refined(rd);
+          else
+            Ss[6 * ns + w] = 1. / rd * (sv9[it][6] * ru + sv9[it][7] * rv + sv9[it][8] * rw);
+        }
+      }
+      __syncthreads();
+    } else if (STG) {
       const int nb = bx * by * bz;
       const double *mt = metric(a, d);
       for (int w = t; w < nb; w += TL::NT) {
@@ -824,7 +976,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       if (MERGE) {
         // ---- viscous flux of the same face
         T G[4], al[3] = {T(0), T(0), T(0)};
-        visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, G, al);
+        VisIn vi;
+        visc_load<LES>(a, d, fk, fj, fi, vi);
+        visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, vi, G, al);
 #pragma unroll
         for (int q = 0; q < 4; q++) Fs[VO + q][f] = G[q];
         if (LES) {
@@ -833,6 +987,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         }
       }
     }
+    if (CLES_STAGE_PREFETCH == 2) prefetch();
+    if (CLES_STAGE_PF_END && d == 2) load_end();
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 5; q++) conv[q] = d == 0 ? Fs[q][hi] - Fs[q][lo] : conv[q] + Fs[q][hi] - Fs[q][lo];
@@ -848,7 +1004,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
         if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
         T G[4], al[3] = {T(0), T(0), T(0)};
-        visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, G, al);
+        if (PF) {
+          if (CLES_STAGE_PF_PART != 3) visc_load<LES, 3 - CLES_STAGE_PF_PART>(a, d, fk, fj, fi, vpre);
+          visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, vpre, G, al);
+        } else {
+          VisIn vi;
+          visc_load<LES>(a, d, fk, fj, fi, vi);
+          visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, vi, G, al);
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) Fs[q][f] = G[q];
         if (LES) {
@@ -884,17 +1047,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     visc[4] = visc[4] + double(e);
   }
   // ---- phase D2: Rhs = aj (Visc - Conv) (MultAj, rhs.c:4611-4629)
-  double rhs[5] = {0., 0., 0., 0., 0.}, qin[5] = {0., 0., 0., 0., 0.};
+  if (!CLES_STAGE_PF_END) load_end();
+  double rhs[5] = {0., 0., 0., 0., 0.};
   if (valid) {
     const bool fluid = !a.has_ib || a.nv[n] < 0.1;
-    const double ajn = a.aj[n];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
       double r = visc[q] - conv[q];
       rhs[q] = fluid ? r * ajn : 0.;
     }
-#pragma unroll
-    for (int q = 0; q < 5; q++) qin[q] = a.q[n + q * np];
   }
   if (a.norms) {
     // every lane of the wave takes part; lanes without a node contribute 0
@@ -922,10 +1083,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   double qn[5];
   if (a.istage == 0) {
 #pragma unroll
-    for (int q = 0; q < 5; q++) qn[q] = a.a_rhs * rhs[q] + a.qold[n + q * np];
+    for (int q = 0; q < 5; q++) qn[q] = a.a_rhs * rhs[q] + qol[q];
   } else {
 #pragma unroll
-    for (int q = 0; q < 5; q++) qn[q] = a.a_old * a.qold[n + q * np] + a.a_rhs * rhs[q] + a.a_cur * qin[q];
+    for (int q = 0; q < 5; q++) qn[q] = a.a_old * qol[q] + a.a_rhs * rhs[q] + a.a_cur * qin[q];
   }
 #pragma unroll
   for (int q = 0; q < 5; q++) a.qo[n + q * np] = qn[q];
