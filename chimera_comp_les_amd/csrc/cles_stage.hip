@@ -62,6 +62,11 @@ constexpr int NT = 256;    // threads per workgroup of the larger tile shapes
 #ifndef CLES_STAGE_PF_END
 #define CLES_STAGE_PF_END 0
 #endif
+// LES forms: load the face densities of the LES energy term at the top of
+// each direction
+#ifndef CLES_STAGE_PF_LESE
+#define CLES_STAGE_PF_LESE 1
+#endif
 constexpr int NSLOT = 64;  // spread of the inf-norm atomics
 
 // Tile of interior nodes owned by one workgroup, with the face counts of its
@@ -888,6 +893,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       }
     };
     if (CLES_STAGE_PREFETCH == 1) prefetch();
+    // face densities of the LES energy term: rho at n, n - st, n + st along d
+    double rle[3];
+    if (CLES_STAGE_PF_LESE && les_e) {
+      const long long st = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
+      rle[0] = a.q[n];
+      rle[1] = a.q[n - st];
+      rle[2] = a.q[n + st];
+    }
     // ---- node data of direction d -> LDS: the state, p and (LES) the
     // contravariant velocity, each computed once per node instead of once per
     // stencil it belongs to
@@ -1026,9 +1039,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       visc[q + 1] = d == 0 ? Fs[VO + q][hi] - Fs[VO + q][lo] : visc[q + 1] + Fs[VO + q][hi] - Fs[VO + q][lo];
     if (les_e) {
       // face densities (r_n + r_n-1)/2, (r_n+1 + r_n)/2 along d
-      const long long st = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
-      const double *r = a.q;
-      const T rl = T((r[n] + r[n - st]) * 0.5), rh = T((r[n + st] + r[n]) * 0.5);
+      double r0, rm, rp;
+      if (CLES_STAGE_PF_LESE) {
+        r0 = rle[0];
+        rm = rle[1];
+        rp = rle[2];
+      } else {
+        const long long st = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
+        r0 = a.q[n];
+        rm = a.q[n - st];
+        rp = a.q[n + st];
+      }
+      const T rl = T((r0 + rm) * 0.5), rh = T((rp + r0) * 0.5);
       const T p0 = rh * T(Fs[VO + 4][hi]), p1 = rh * T(Fs[VO + 5][hi]), p2 = rh * T(Fs[VO + 6][hi]);
       E0 = d == 0 ? p0 - rl * T(Fs[VO + 4][lo]) : E0 + p0 - rl * T(Fs[VO + 4][lo]);
       E1 = d == 0 ? p1 - rl * T(Fs[VO + 5][lo]) : E1 + p1 - rl * T(Fs[VO + 5][lo]);
