@@ -95,9 +95,16 @@ __device__ __forceinline__ double sabs_of(const double g[9]) {
 // filter uses (S_ab = (du_a/dx_b + du_b/dx_a)/2, les.c:520-522) and |S| with
 // the trace term removed (les.c:268-274).  Also stores the velocity lUcat
 // holds once the loop has finished, read by the test filter and by nu_t.
-__global__ __launch_bounds__(256) void les_grad_kernel(LesArgs a) {
+// Node-range kernels: each XCD (blockIdx % 8) walks a contiguous range of
+// blocks, so the k +- 1 neighbours of its nodes stay in its own L2.
+__device__ __forceinline__ long long xcd_node(int per_xcd) {
+  const int b = blockIdx.x;
+  return (long long)((b & 7) * per_xcd + (b >> 3)) * blockDim.x + threadIdx.x;
+}
+
+__global__ __launch_bounds__(256) void les_grad_kernel(LesArgs a, int per_xcd) {
   const Grid &g = a.g;
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = xcd_node(per_xcd);
   if (n >= (long long)g.mx * g.my * g.mz) return;
   const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
   const long long np = g.np;
@@ -438,9 +445,9 @@ __global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs a, int ntx, int
 }
 
 // nu_t = Cs Delta^2 |S| (les.c:1536-1573)
-__global__ __launch_bounds__(256) void les_nut_kernel(LesArgs a) {
+__global__ __launch_bounds__(256) void les_nut_kernel(LesArgs a, int per_xcd) {
   const Grid &g = a.g;
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = xcd_node(per_xcd);
   if (n >= (long long)g.mx * g.my * g.mz) return;
   const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
   if (!interior(g, k, j, i) || nvat(a, n) > 0.1) {
@@ -487,6 +494,8 @@ int launch_les(cles_ctx *c) {
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
   const int bs = 256;
   const unsigned gN = (unsigned)((N + bs - 1) / bs);
+  const int per_xcd = (int)((gN + 7) / 8);  // blocks per XCD of the node-range kernels
+  const unsigned gX = 8u * per_xcd;
   if (c->ucat_les_mixed)
     return set_error(c, CLES_ESTATE, "LES model called twice without a right-hand-side evaluation in between");
   // les.c:139 tests the global tistart, which main.c:890-891 has incremented
@@ -502,11 +511,11 @@ int launch_les(cles_ctx *c) {
     // the last CalcUCat
     hipLaunchKernelGGL(fill_kernel, dim3(gN), dim3(bs), 0, c->stream, c->cs, N, early ? 0.0 : 0.03);
     CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(les_grad_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
+    hipLaunchKernelGGL(les_grad_kernel, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
     CLES_HIP(c, hipGetLastError());
   } else {
     a.ucat_updated = 1;
-    hipLaunchKernelGGL(les_grad_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
+    hipLaunchKernelGGL(les_grad_kernel, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
     CLES_HIP(c, hipGetLastError());
     {
       const int ntx = (c->g.mx - 2 + FTX - 1) / FTX, nty = (c->g.my - 2 + FTY - 1) / FTY,
@@ -530,7 +539,7 @@ int launch_les(cles_ctx *c) {
     a.cs = c->cs;
     c->ucat_les_mixed = 1;
   }
-  hipLaunchKernelGGL(les_nut_kernel, dim3(gN), dim3(bs), 0, c->stream, a);
+  hipLaunchKernelGGL(les_nut_kernel, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
   CLES_HIP(c, hipGetLastError());
   return CLES_OK;
 }
