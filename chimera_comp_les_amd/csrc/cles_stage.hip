@@ -9,18 +9,20 @@
 // in registers, so the only HBM traffic is the stage input, the metrics and
 // the stage output.
 //
-// Work decomposition: a workgroup of 256 threads owns a tile of interior
-// nodes (one node per thread; 6x6x6 by default, five other shapes).  The
-// primitive variables (u, v, w, p/rho) of the tile plus a one-node halo are
-// staged in LDS first.  Then, one direction at a time: the convective fluxes
-// of the direction's faces (6x6x6: 7*6*6 = 252, one pass of the workgroup)
-// go to an LDS face buffer and each node adds F(hi) - F(lo); the viscous
-// fluxes of the same faces follow through the same buffer.  For the FP64
-// central flux of the LES form the direction's node data (state, p and the
-// contravariant velocity) are staged in LDS as well, so each node's division
-// is done once.  Faces are computed once per workgroup instead of twice per
-// node.  Tiles are numbered so that each XCD (blockIdx % 8) receives a
-// contiguous slab of tiles and keeps its neighbours in its own L2.
+// Work decomposition: a workgroup owns a tile of interior nodes (one node
+// per thread; 8x4x3 nodes in 128 threads by default, seven other shapes).
+// The primitive variables (u, v, w, p/rho) of the tile plus a one-node halo
+// are staged in LDS first.  Then, one direction at a time: the convective
+// fluxes of the direction's faces (8x4x3: at most 128, one pass of the
+// workgroup) go to an LDS face buffer and each node adds F(hi) - F(lo); the
+// viscous fluxes of the same faces follow, in the same pass or through the
+// same buffer.  For the central flux of the LES forms the direction's node
+// data (state, p and the contravariant velocity) are staged in LDS as well,
+// so each node's division is done once.  Faces are computed once per
+// workgroup instead of twice per node.  Tiles are numbered so that each XCD
+// (blockIdx % 8) receives a contiguous slab of tiles and keeps its
+// neighbours in its own L2.  The kernel is bound by load latency, so loads
+// are issued in groups and ahead of the phases that use them.
 #include <cstring>
 #include <type_traits>
 
@@ -38,7 +40,7 @@ constexpr int NT = 256;    // threads per workgroup of the larger tile shapes
 #endif
 // convective and viscous flux of a face computed in one pass (one barrier
 // pair per direction instead of two): 0 never, 1 always, 2 all forms but the
-// FP64 LES form (measured faster there as two passes)
+// LES forms (measured faster there as two passes)
 #ifndef CLES_STAGE_MERGE
 #define CLES_STAGE_MERGE 2
 #endif
@@ -80,7 +82,7 @@ struct Tile {
   static constexpr int NF = NFI + NFJ + NFK;
   static constexpr int WX = X + 2, WY = Y + 2, WZ = Z + 2, NW = WX * WY * WZ;
 };
-// tile shapes the launcher can pick (index = cles_ctx::tile, default 1)
+// tile shapes the launcher can pick (index = cles_ctx::tile, default 7)
 typedef Tile<32, 4, 2> TileA;
 typedef Tile<16, 4, 4> TileB;
 typedef Tile<16, 8, 2> TileC;
@@ -775,7 +777,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, WX = TL::WX, WY = TL::WY, NW = TL::NW;
   // face buffer: the convective flux (5) and, computed in the same pass, the
   // viscous flux (4) and the LES stresses (3) of each face of one direction
-  constexpr bool MERGE = VISC && (CLES_STAGE_MERGE == 1 || (CLES_STAGE_MERGE == 2 && !(LES && sizeof(T) == 8)));
+  constexpr bool MERGE = VISC && (CLES_STAGE_MERGE == 1 || (CLES_STAGE_MERGE == 2 && !LES));
   constexpr int FV = VISC ? (LES ? 7 : 4) : 0;
   constexpr int FC = MERGE ? 5 + FV : (VISC ? (LES ? 7 : 5) : 5);
   constexpr int VO = MERGE ? 5 : 0;  // first viscous slot
@@ -785,10 +787,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   constexpr int NCH = LES ? 7 : 6;
   constexpr int NS0 = (TX + 4) * TY * TZ, NS1 = TX * (TY + 4) * TZ, NS2 = TX * TY * (TZ + 4);
   constexpr int NSM = NS0 > NS1 ? (NS0 > NS2 ? NS0 : NS2) : (NS1 > NS2 ? NS1 : NS2);
-  // staging pays where it saves the per-stencil-node division of the FP64
-  // central flux and two workgroups still fit in a CU's LDS; elsewhere the
-  // extra LDS costs more occupancy than the loads it saves
-  constexpr bool STG = LES && sizeof(T) == 8 &&
+  // staging pays where it saves the per-stencil-node division of the
+  // central flux (LES forms) and two workgroups still fit in a CU's LDS;
+  // elsewhere the extra LDS costs more occupancy than the loads it saves
+  constexpr bool STG = LES &&
                        8 * (FC * NFD + NCH * NSM) + (VISC ? (int)sizeof(T) * 4 * NW : 0) <= 80 * 1024;
   __shared__ double Fs[FC][NFD];
   __shared__ T Ws[VISC ? 4 * NW : 1];
