@@ -204,7 +204,7 @@ size_t ctx_footprint(const cles_params *p) {
   size_t planes = 3 + 9 + 1 + 1 + 1 + 15 + 3 + 5;  // coor, metrics, aj, nvert, fil2, Q x3, P x3, rhs
   if (p->les) {
     planes += 1 + 3 + 3;  // tfil2, cs, cs_alt, nut, ua
-    if (p->les != 1 && p->les != 10) planes += 7 + 2 + 3;  // strain rates, LM, MM, LM/27, MM/27, 1/aj
+    if (p->les != 1 && p->les != 10) planes += 7 + 2 + 3 + 1;  // strain rates, LM, MM, LM/27, MM/27, 1/aj, |S|
   }
   if (p->averaging) planes += 12;  // Do_averaging sums
   const size_t xfer = (size_t)((p->mx + 4) * (long long)(p->my + 4) * (p->mz + 4)) * 5;
@@ -291,6 +291,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
       TRY(dalloc(c, &c->lmq, np));
       TRY(dalloc(c, &c->mmq, np));
       TRY(dalloc(c, &c->ivol, np));
+      TRY(dalloc(c, &c->sabs, np));
     }
   }
   if (p->averaging) TRY(dalloc(c, &c->avg, 12 * np));
@@ -313,7 +314,7 @@ void cles_destroy(cles_ctx *c) {
   if (!c) return;
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->ua, c->qsnap, c->tfil2, c->lmq, c->mmq, c->ivol, c->avg};
+                    c->fil2, c->ua, c->qsnap, c->tfil2, c->lmq, c->mmq, c->ivol, c->sabs, c->avg};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   ibm_free(c);

@@ -203,6 +203,7 @@ struct cles_ctx {
   double *ua;      // 3 planes: velocity held by lUcat after the gradient loop
   double *lm, *mm;
   double *lmq, *mmq;  // LM/27, MM/27: the terms of the 27-point means
+  double *sabs;       // |S| of the velocity after the gradient loop, for nu_t
   double *ivol;       // 1/aj, the smoothing weights
   cles::SigmaSpec sig;
   // reductions

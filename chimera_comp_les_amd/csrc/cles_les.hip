@@ -33,6 +33,7 @@ struct LesArgs {
   double *ua;    // 3 planes: lUcat after the gradient loop
   double *lm, *mm;
   double *lmq, *mmq;    // LM/27, MM/27
+  double *sabs;         // |S| for nu_t, formed by the test filter
   const double *ivol;   // 1/aj
   double *cs, *cs_out, *nut;
   const double *fil2;   // per-node (pow(1/aj, 1/3))^2, formed once at setup
@@ -222,6 +223,22 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a, int ntx, int
           s_rvw = fma(w, rv * u2, s_rvw);
           s_rww = fma(w, rw * u2, s_rww);
         }
+    // |S| of the velocity lUcat holds after the gradient loop, the input of
+    // nu_t (les.c:1536-1573): its six neighbours are in the window
+    double up_i[3], um_i[3], up_j[3], um_j[3], up_k[3], um_k[3];
+    constexpr int sj = FBX, sk = FBX * FBY;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      up_i[c] = B[4 + c][c0 + 1];
+      um_i[c] = B[4 + c][c0 - 1];
+      up_j[c] = B[4 + c][c0 + sj];
+      um_j[c] = B[4 + c][c0 - sj];
+      up_k[c] = B[4 + c][c0 + sk];
+      um_k[c] = B[4 + c][c0 - sk];
+    }
+    double gr[9];
+    grad9(a, n0, up_i, um_i, up_j, um_j, up_k, um_k, gr);
+    a.sabs[n0] = sabs_of(gr);
   }
   __syncthreads();
   // ---- pass 2: strain rates and |S| as the filter sees them (rho stays in B[0])
@@ -473,6 +490,19 @@ __global__ __launch_bounds__(256) void les_nut_kernel(LesArgs a, int per_xcd) {
   a.nut[n] = a.cs[n] * a.fil2[n] * Sabs;
 }
 
+// nu_t from the |S| the test filter formed (same product as les_nut_kernel)
+__global__ __launch_bounds__(256) void les_nut_stored_kernel(LesArgs a, int per_xcd) {
+  const Grid &g = a.g;
+  const long long n = xcd_node(per_xcd);
+  if (n >= (long long)g.mx * g.my * g.mz) return;
+  const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
+  if (!interior(g, k, j, i) || nvat(a, n) > 0.1) {
+    a.nut[n] = 0.0;
+    return;
+  }
+  a.nut[n] = a.cs[n] * a.fil2[n] * a.sabs[n];
+}
+
 __global__ __launch_bounds__(256) void fill_kernel(double *x, long long n, double v) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) x[t] = v;
@@ -488,7 +518,7 @@ int launch_les(cles_ctx *c) {
   a.qs = c->ucat_src >= 0 ? c->Q[c->ucat_src] : nullptr;
   a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert;
   a.has_ib = c->has_ib;
-  a.grad = c->grad; a.lm = c->lm; a.mm = c->mm; a.lmq = c->lmq; a.mmq = c->mmq; a.ivol = c->ivol; a.ua = c->ua; a.fil2 = c->fil2; a.tfil2 = c->tfil2;
+  a.grad = c->grad; a.lm = c->lm; a.mm = c->mm; a.lmq = c->lmq; a.mmq = c->mmq; a.ivol = c->ivol; a.sabs = c->sabs; a.ua = c->ua; a.fil2 = c->fil2; a.tfil2 = c->tfil2;
   a.cs = c->cs; a.cs_out = c->cs_alt; a.nut = c->nut;
   a.les = p.les; a.testfilter_ik = p.testfilter_ik; a.max_cs = p.max_cs;
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
@@ -539,7 +569,10 @@ int launch_les(cles_ctx *c) {
     a.cs = c->cs;
     c->ucat_les_mixed = 1;
   }
-  hipLaunchKernelGGL(les_nut_kernel, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
+  if (a.ucat_updated)
+    hipLaunchKernelGGL(les_nut_stored_kernel, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
+  else
+    hipLaunchKernelGGL(les_nut_kernel, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
   CLES_HIP(c, hipGetLastError());
   return CLES_OK;
 }
