@@ -127,6 +127,29 @@ def test_steps_tgv_les(fluxorder):
     assert state_err(Qg, Qc) <= STATE_TOL
 
 
+@pytest.mark.parametrize("les", [1, 3, 10])
+def test_steps_tgv_les_modes(les):
+    """The other LES switches: 1 = constant Cs (no test filter; nu_t from its
+    own velocity gradients), 3 = dynamic model without the 27-point means of
+    LM and MM, 10 = Cs = 0 (the early-out of les.c:139)."""
+    cfg, coor, Q = cases.tgv_case(N=12, les=les)
+    Q = perturbed(Q, 7, amp=1e-2)
+    o, g = _pair(cfg, coor, Q)
+    for s in range(3):
+        o.step(1)
+        g.flow_step()
+        cso, nuo, _ = o.les_fields()
+        csg, nug, _ = g.les_fields()
+        assert np.abs(csg - cso).max() <= 1e-12 * max(np.abs(cso).max(), 1e-300), s
+        assert np.abs(nug - nuo).max() <= 1e-12 * max(np.abs(nuo).max(), 1e-300), s
+    if les != 10:
+        assert np.abs(nuo).max() > 0
+    Qc, _ = o.state()
+    Qg, _ = g.state()
+    g.close()
+    assert state_err(Qg, Qc) <= STATE_TOL
+
+
 def test_norms_match():
     cfg, coor, Q = small_vortex(N=10)
     o, g = _pair(cfg, coor, Q)
