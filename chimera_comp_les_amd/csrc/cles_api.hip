@@ -127,7 +127,7 @@ int check_params(const cles_params *p) {
   if (p->fluxorder < 1 || p->fluxorder > 3) return set_error(nullptr, CLES_EINVAL, "fluxorder must be 1, 2 or 3");
   if (p->eno_order != 2 && p->eno_order != 3) return set_error(nullptr, CLES_EINVAL, "ENOorder must be 2 or 3");
   if (p->calc_sigma) return set_error(nullptr, CLES_EINVAL, "-calc_sigma (shock sensor) is not on this path");
-  if (p->twoD) return set_error(nullptr, CLES_EINVAL, "-twoD is not on this path");
+  if (p->twoD < 0 || p->twoD > 3) return set_error(nullptr, CLES_EINVAL, "-twoD must be 0, 1, 2 or 3");
   if (p->precision != CLES_PREC_FP64 && p->precision != CLES_PREC_FP32BULK)
     return set_error(nullptr, CLES_EINVAL, "unknown precision %d", p->precision);
   if (p->les && p->invicid) return set_error(nullptr, CLES_EINVAL, "LES requires a viscous run (-inv 0)");

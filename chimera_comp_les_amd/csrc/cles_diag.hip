@@ -80,6 +80,7 @@ struct AvgArgs {
   long long n, np;
   const double *q, *p, *qu;  // state, pressure, CalcUCat input (or null)
   double *s;                 // 12 planes
+  int twoD;                  // 1: CalcUCat sets u = 0 (rhs.c:1349)
 };
 
 __global__ __launch_bounds__(256) void averaging_kernel(AvgArgs a) {
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(256) void averaging_kernel(AvgArgs a) {
   double u = 0., v = 0., w = 0.;
   if (a.qu) {
     const double r = a.qu[t];
-    u = a.qu[np + t] / r;
+    u = a.twoD == 1 ? 0. : a.qu[np + t] / r;
     v = a.qu[2 * np + t] / r;
     w = a.qu[3 * np + t] / r;
   }
@@ -116,6 +117,7 @@ int launch_averaging(cles_ctx *c) {
   a.p = c->P[c->cur];
   a.qu = c->ucat_src >= 0 ? c->Q[c->ucat_src] : nullptr;
   a.s = c->avg;
+  a.twoD = c->p.twoD;
   hipLaunchKernelGGL(averaging_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, c->stream, a);
   CLES_HIP(c, hipGetLastError());
   return CLES_OK;

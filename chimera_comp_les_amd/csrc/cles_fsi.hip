@@ -29,6 +29,7 @@ struct ForceArgs {
   const double *nf;     // 3 planes of triangles
   int ne;
   const double *q;      // CalcUCat input (or null: lUcat still zero)
+  int twoD;             // 1: CalcUCat sets u = 0 (rhs.c:1349)
   const double *p, *nvert, *coor, *csi, *eta, *zet, *aj;
   double rei;
   double *rec;          // kRec per list node
@@ -41,7 +42,7 @@ __device__ __forceinline__ void ucat_at(const ForceArgs &a, long long n, double 
   }
   const long long np = a.g.np;
   const double r = a.q[n];
-  u[0] = a.q[np + n] / r;
+  u[0] = a.twoD == 1 ? 0. : a.q[np + n] / r;
   u[1] = a.q[2 * np + n] / r;
   u[2] = a.q[3 * np + n] / r;
 }
@@ -299,6 +300,7 @@ extern "C" int cles_ibm_forces(cles_ctx *c, const double center[3], double out[C
   // lUcat as the body sees it at the start of a step: the velocity of the
   // last CalcUCat (its stage input), zero before the first
   a.q = c->ucat_src >= 0 ? c->Q[c->ucat_src] : nullptr;
+  a.twoD = c->p.twoD;
   a.p = c->P[c->cur];
   a.nvert = c->nvert; a.coor = c->coor; a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj;
   a.rei = 1. / c->p.ren;

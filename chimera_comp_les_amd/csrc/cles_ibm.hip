@@ -407,7 +407,7 @@ struct StencilArgs {
   Grid g;
   const double *coor;
   Surf s;
-  int n_ib, iso;
+  int n_ib, iso, twoD;
   const int *node;
   int *cell, *imode, *don;  // don: 9 planes (i1 j1 k1 i2 j2 k2 i3 j3 k3)
   double *info;             // 11 planes: pmin xyz, d_s, d_i, cs1..3, cr1..3
@@ -555,8 +555,11 @@ __global__ __launch_bounds__(128) void stencil_kernel(StencilArgs a) {
   }
   const double vals[11] = {pmin.x, pmin.y, pmin.z, dmin, d_i, cs1, cs2, cs3, cr1, cr2, cr3};
   for (int q = 0; q < 11; q++) a.info[q * nib + m] = vals[q];
-  // normal of the interpolation (ibm_iso.c:2665-2689), twoD = 0
-  double ux = p.x - pmin.x, uy = p.y - pmin.y, uz = p.z - pmin.z;
+  // normal of the interpolation (ibm_iso.c:2662-2689): -twoD drops one
+  // component before the normalisation
+  double ux = a.twoD == 1 ? 0. : p.x - pmin.x;
+  double uy = a.twoD == 2 ? 0. : p.y - pmin.y;
+  double uz = a.twoD == 3 ? 0. : p.z - pmin.z;
   const double dr = sqrt(ux * ux + uy * uy + uz * uz);
   if (dr < 0.000000001) {
     ux = s.nf[cmin]; uy = s.nf[ne + cmin]; uz = s.nf[2 * ne + cmin];
@@ -571,7 +574,7 @@ __global__ __launch_bounds__(128) void stencil_kernel(StencilArgs a) {
 // ------------------------------------------------------------ interpolation
 
 struct InterpArgs {
-  int n_ib, iso, invicid, add_dundt, sweep;
+  int n_ib, iso, invicid, add_dundt, sweep, twoD;
   long long np;
   const int *node, *cell;
   const long long *dn;
@@ -626,6 +629,10 @@ __global__ __launch_bounds__(256) void interp_prep_kernel(InterpArgs a) {
     Ubx = sb / sc * u_c; Uby = sb / sc * v_c; Ubz = sb / sc * w_c;
     Ubx += Uax; Uby += Uay; Ubz += Uaz;
   }
+  // 2D constraint, ibm_iso.c:2744-2746
+  if (a.twoD == 1) Ubx = 0.;
+  if (a.twoD == 2) Uby = 0.;
+  if (a.twoD == 3) Ubz = 0.;
   const double pint = (a.p[d1] * sk1 + a.p[d2] * sk2 + a.p[d3] * sk3);
   const double gm2 = a.gamma * a.Ma * a.Ma;
   double Tb = 0, X = 0, Lsc = 0, rho = 0;
@@ -807,6 +814,7 @@ int launch_ibm_interp(cles_ctx *c, int buf, int qold_buf, int add_dundt) {
   a.iso = c->p.isothermal;
   a.invicid = c->p.invicid;
   a.add_dundt = add_dundt;
+  a.twoD = c->p.twoD;
   a.np = c->g.np;
   a.node = b->node; a.cell = b->cell; a.dn = b->dn; a.info = b->info; a.nrm = b->nrm;
   a.s = Surf{b->nv, b->ne, b->vx, b->vy, b->vz, b->tv, b->nf, b->qv, b->rad};
@@ -1030,7 +1038,7 @@ int cles_ibm_search(cles_ctx *c, int n_v, const double *x_bp, const double *y_bp
                        nib, b->node);
     TRY_HIP(hipGetLastError());
     StencilArgs sa;
-    sa.g = g; sa.coor = c->coor; sa.s = s; sa.n_ib = nib; sa.iso = iso; sa.node = b->node;
+    sa.g = g; sa.coor = c->coor; sa.s = s; sa.n_ib = nib; sa.iso = iso; sa.twoD = c->p.twoD; sa.node = b->node;
     sa.cell = b->cell; sa.imode = b->imode; sa.don = b->don; sa.info = b->info; sa.dn = b->dn; sa.nrm = b->nrm;
     sa.flags = d_flags; sa.first_bad = d_bad;
     hipLaunchKernelGGL(stencil_kernel, dim3((unsigned)((nib + 127) / 128)), dim3(128), 0, c->stream, sa);

@@ -39,6 +39,7 @@ struct LesArgs {
   const double *fil2;   // per-node (pow(1/aj, 1/3))^2, formed once at setup
   const double *tfil2;  // per-node squared test-filter width
   int les, testfilter_ik, ucat_updated;
+  int twoD;  // 1: the last CalcUCat set u = 0 (rhs.c:1349); the loop's own velocities keep it
   double max_cs;
 };
 
@@ -57,8 +58,12 @@ __device__ __forceinline__ void vel(const double *q, long long n, long long np, 
   u[2] = q[n + 3 * np] / r;
 }
 __device__ __forceinline__ void u_stale(const LesArgs &a, long long n, double u[3]) {
-  if (a.qs) vel(a.qs, n, a.g.np, u);
-  else u[0] = u[1] = u[2] = 0.0;
+  if (a.qs) {
+    vel(a.qs, n, a.g.np, u);
+    if (a.twoD == 1) u[0] = 0.0;
+  } else {
+    u[0] = u[1] = u[2] = 0.0;
+  }
 }
 // Compute_du_center + Compute_du_dxyz (les.c:31-63)
 __device__ __forceinline__ void grad9(const LesArgs &a, long long n, const double up_i[3], const double um_i[3],
@@ -520,7 +525,7 @@ int launch_les(cles_ctx *c) {
   a.has_ib = c->has_ib;
   a.grad = c->grad; a.lm = c->lm; a.mm = c->mm; a.lmq = c->lmq; a.mmq = c->mmq; a.ivol = c->ivol; a.sabs = c->sabs; a.ua = c->ua; a.fil2 = c->fil2; a.tfil2 = c->tfil2;
   a.cs = c->cs; a.cs_out = c->cs_alt; a.nut = c->nut;
-  a.les = p.les; a.testfilter_ik = p.testfilter_ik; a.max_cs = p.max_cs;
+  a.les = p.les; a.testfilter_ik = p.testfilter_ik; a.max_cs = p.max_cs; a.twoD = p.twoD;
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
   const int bs = 256;
   const unsigned gN = (unsigned)((N + bs - 1) / bs);

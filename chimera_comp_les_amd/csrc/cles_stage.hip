@@ -113,6 +113,7 @@ struct StageArgs {
   // deviations from it, the reference parts of the fluxes stay in double
   double rho_ref, p_ref, e_ref, h_ref, t_ref;
   int suth;
+  int twoD;  // -twoD 1: no xi-direction fluxes, u = 0 in CalcUCat, no rhoU residual
   SigmaSpec sig;
   double a_rhs, a_old, a_cur;
   int istage;
@@ -853,7 +854,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         const int w = t + it * TL::NT;
         if (!ok[it]) continue;
         const double r = qv[it][0];
-        Ws[w] = T(qv[it][1] / r);
+        Ws[w] = a.twoD == 1 ? T(0) : T(qv[it][1] / r);  // CalcUCat, rhs.c:1349
         Ws[NW + w] = T(qv[it][2] / r);
         Ws[2 * NW + w] = T(qv[it][3] / r);
         Ws[3 * NW + w] = T(qv[it][4] / r - a.t_ref);
@@ -865,7 +866,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         if (ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
         long long m = nid(g, kk, jj, ii);
         double r = a.q[m];
-        Ws[w] = T(a.q[m + np] / r);
+        Ws[w] = a.twoD == 1 ? T(0) : T(a.q[m + np] / r);
         Ws[NW + w] = T(a.q[m + 2 * np] / r);
         Ws[2 * NW + w] = T(a.q[m + 3 * np] / r);
         Ws[3 * NW + w] = T(a.p[m] / r - a.t_ref);  // t_ref = 0 in FP64: only gradients of p/rho are used
@@ -971,11 +972,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       }
       __syncthreads();
     }
+    // -twoD 1 computes no xi face (rhs.c:1487-1714, 2646-2814, 3465-3706):
+    // their fluxes and LES stresses are the zeros of fresh Vecs
+    const bool skip_d = d == 0 && a.twoD == 1;
     // ---- convective fluxes of direction d -> LDS
     for (int f = t; f < nfd; f += TL::NT) {
       const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
       const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
       if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
+      if (skip_d) {
+#pragma unroll
+        for (int q = 0; q < FC; q++) Fs[q][f] = 0.0;
+        continue;
+      }
       Sv sv{nullptr, ns, 0, 0, false};
       if (STG) {
         sv.on = true;
@@ -1018,6 +1027,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
         const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
         if (!(fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2)) continue;
+        if (skip_d) {
+#pragma unroll
+          for (int q = 0; q < FC; q++) Fs[q][f] = 0.0;
+          continue;
+        }
         T G[4], al[3] = {T(0), T(0), T(0)};
         if (PF) {
           if (CLES_STAGE_PF_PART != 3) visc_load<LES, 3 - CLES_STAGE_PF_PART>(a, d, fk, fj, fi, vpre);
@@ -1059,6 +1073,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       E2 = d == 0 ? p2 - rl * T(Fs[VO + 6][lo]) : E2 + p2 - rl * T(Fs[VO + 6][lo]);
     }
     __syncthreads();
+  }
+  if (a.twoD == 1) {  // rhs.c:2135-2140, 3168-3173, 4215-4220
+    conv[1] = 0.;
+    visc[1] = 0.;
   }
   if (LES && a.ratio != 0.0 && valid) {
     conv[3] = conv[3] - a.ratio / a.aj[n] * a.q[n] / a.dt;
@@ -1217,6 +1235,7 @@ void fill_stage_args(cles_ctx *c, int in, int out, int istage, const double coef
   a.h_ref = a.e_ref + a.p_ref;
   a.t_ref = f32 ? c->ref_p / c->ref_rho : 0.0;
   a.suth = p.sutherland;
+  a.twoD = p.twoD;
   a.sig = c->sig;
   a.a_rhs = coef[0]; a.a_old = coef[1]; a.a_cur = coef[2];
   a.istage = istage;

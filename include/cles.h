@@ -71,7 +71,10 @@ typedef struct cles_params {
   int isothermal;        /* -isothermal                                       */
   int calc_sigma;        /* -calc_sigma (must be 0 on this path)              */
   int abc;               /* -ABC                                              */
-  int twoD;              /* -twoD (must be 0 on this path)                    */
+  int twoD;              /* -twoD: 0, or 1/2/3 = drop x/y/z (1: no xi fluxes,
+                          * u = 0, no rhoU residual, rhs.c:1349, 1487, 2135,
+                          * 2646, 3168, 3465, 4215; all: IB normal and
+                          * surface velocity, ibm_iso.c:2662, 2744)          */
   int testfilter_ik;     /* -testfilter_ik                                    */
   int tistart;           /* -rstart value (0 = fresh start); the time loop
                             sees it incremented (main.c:890-891)            */

@@ -60,7 +60,7 @@ typedef struct {
   int isothermal;          /* -isothermal                                    */
   int calc_sigma;          /* -calc_sigma (rhs.c:3335)                       */
   int abc;                 /* -ABC (rhs.c:3307)                              */
-  int twoD;                /* -twoD, read by FormBCS (rhs.c:512)             */
+  int twoD;                /* -twoD 0..3 (rhs.c:512, 1339, 1413, 2585, 3245; ibm_iso.c:2619) */
   int testfilter_ik;       /* -testfilter_ik (main.c:67,788)                 */
   int tistart;             /* time step the run starts from (main.c:890)     */
   int rstart;              /* restart flag (main.c:810)                      */

@@ -602,12 +602,15 @@ void orc_form_bcs(orc_ctx *c, double alfa) {
           L1 = Lam1 * AJ(k - 1, j, i) * ZET(k - 1, j, i).z *
                (a_sound * a_sound * (LQ(k - 1, j, i).rho - LQ(k - 2, j, i).rho) -
                 (LP(k, j, i) - LP(k - 1, j, i)));
-          if (c->p.twoD != 1)
+          /* -TwoD (the global of main.c:36, 749), not -twoD: a different
+           * option, not on this path, so it keeps its default 0 here */
+          const int TwoD = 0;
+          if (TwoD != 1)
             L2 = Lam2 * AJ(k - 1, j, i) * ZET(k - 1, j, i).z *
                  (LQ(k - 1, j, i).rhoU / Qg(k - 1, j, i).rho - LQ(k - 2, j, i).rhoU / Qg(k - 1, j, i).rho);
           else
             L2 = 0.;
-          if (c->p.twoD != 2)
+          if (TwoD != 2)
             L3 = Lam3 * AJ(k - 1, j, i) * ZET(k - 1, j, i).z *
                  (LQ(k - 1, j, i).rhoV / Qg(k - 1, j, i).rho - LQ(k - 2, j, i).rhoV / Qg(k - 2, j, i).rho);
           else

@@ -610,7 +610,11 @@ void orc_ibm_interpolation(orc_ctx *c, int add_dundt) {
       const double sk1 = b->cr1, sk2 = b->cr2, sk3 = b->cr3;
       const double cs1 = b->cs1, cs2 = b->cs2, cs3 = b->cs3;
       const orc_v3 x = c->coor[g];
-      double nfx = x.x - b->pmin[0], nfy = x.y - b->pmin[1], nfz = x.z - b->pmin[2];
+      /* -twoD drops one component of the normal, ibm_iso.c:2662-2664 */
+      const int twoD = P->twoD;
+      double nfx = twoD == 1 ? 0. : x.x - b->pmin[0];
+      double nfy = twoD == 2 ? 0. : x.y - b->pmin[1];
+      double nfz = twoD == 3 ? 0. : x.z - b->pmin[2];
       const double dr = sqrt(nfx * nfx + nfy * nfy + nfz * nfz);
       if (dr < 0.000000001) {
         if (ni >= 0) { nfx = c->nf_x[ni]; nfy = c->nf_y[ni]; nfz = c->nf_z[ni]; }
@@ -639,11 +643,15 @@ void orc_ibm_interpolation(orc_ctx *c, int add_dundt) {
       Uc.x = (q1->rhoU / q1->rho * sk1 + q2->rhoU / q2->rho * sk2 + q3->rhoU / q3->rho * sk3);
       Uc.y = (q1->rhoV / q1->rho * sk1 + q2->rhoV / q2->rho * sk2 + q3->rhoV / q3->rho * sk3);
       Uc.z = (q1->rhoW / q1->rho * sk1 + q2->rhoW / q2->rho * sk2 + q3->rhoW / q3->rho * sk3);
-      const orc_v3 Ub = P->invicid ? freeslip(sc, sb, Ua, Uc, nfx, nfy, nfz) : noslip(sc, sb, Ua, Uc);
+      orc_v3 Ub = P->invicid ? freeslip(sc, sb, Ua, Uc, nfx, nfy, nfz) : noslip(sc, sb, Ua, Uc);
       double cv1 = c->lP[l1], cv2 = c->lP[l2], cv3 = c->lP[l3];
       double *p = &c->P[g];
       orc_q *qq = &c->Q[g];
       *p = (cv1 * sk1 + cv2 * sk2 + cv3 * sk3);
+      /* 2D constraint, ibm_iso.c:2744-2746 */
+      if (twoD == 1) Ub.x = 0.;
+      if (twoD == 2) Ub.y = 0.;
+      if (twoD == 3) Ub.z = 0.;
       if (iso == 2) {
         cv1 = q1->rho; cv2 = q2->rho; cv3 = q3->rho;
         const double rhoc = (cv1 * sk1 + cv2 * sk2 + cv3 * sk3);

@@ -199,7 +199,11 @@ static void residual(const orc_ctx *c, orc_q *out, int les_forcing) {
         orc_q *o = &out[GI(c, k, j, i)];
         if (c->lnvert[n] < BLANK) {
           o->rho = f1[n].rho - f1[n - sx].rho + f2[n].rho - f2[n - sy].rho + f3[n].rho - f3[n - sz].rho;
-          o->rhoU = f1[n].rhoU - f1[n - sx].rhoU + f2[n].rhoU - f2[n - sy].rhoU + f3[n].rhoU - f3[n - sz].rhoU;
+          /* -twoD 1: no rhoU residual, rhs.c:3168-3173, 4215-4220 */
+          if (c->p.twoD != 1)
+            o->rhoU = f1[n].rhoU - f1[n - sx].rhoU + f2[n].rhoU - f2[n - sy].rhoU + f3[n].rhoU - f3[n - sz].rhoU;
+          else
+            o->rhoU = 0.;
           o->rhoV = f1[n].rhoV - f1[n - sx].rhoV + f2[n].rhoV - f2[n - sy].rhoV + f3[n].rhoV - f3[n - sz].rhoV;
           o->rhoW = f1[n].rhoW - f1[n - sx].rhoW + f2[n].rhoW - f2[n - sy].rhoW + f3[n].rhoW - f3[n - sz].rhoW;
           o->rhoE = f1[n].rhoE - f1[n - sx].rhoE + f2[n].rhoE - f2[n - sy].rhoE + f3[n].rhoE - f3[n - sz].rhoE;
@@ -214,13 +218,29 @@ static void residual(const orc_ctx *c, orc_q *out, int les_forcing) {
       }
 }
 
+/* -twoD 1 skips the xi faces (rhs.c:1487-1714, 2646-2814, 3465-3706): their
+ * flux Vecs are fresh VecDuplicates, zero where no face writes them. */
+static void zero_xi_faces(orc_ctx *c, orc_q *F, orc_v3 *al) {
+  #pragma omp parallel for schedule(static)
+  for (int k = 1; k < c->mz - 1; k++)
+    for (int j = 1; j < c->my - 1; j++)
+      for (int i = 0; i < c->mx - 1; i++) {
+        const size_t n = LI(c, k, j, i);
+        F[n].rho = F[n].rhoU = F[n].rhoV = F[n].rhoW = F[n].rhoE = 0.;
+        if (al) al[n].x = al[n].y = al[n].z = 0.;
+      }
+}
+
 /* formConvection, rhs.c:2570-3224 (non-LES). Writes c->conv. */
 void orc_form_convection(orc_ctx *c) {
   const int mx = c->mx, my = c->my, mz = c->mz;
+  if (c->p.twoD == 1) zero_xi_faces(c, c->fp1, NULL);
+  else {
   #pragma omp parallel for schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 1; j < my - 1; j++)
       for (int i = 0; i < mx - 1; i++) face_conv(c, 0, k, j, i, &c->fp1[LI(c, k, j, i)]);
+  }
   #pragma omp parallel for schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 0; j < my - 1; j++)
@@ -407,10 +427,13 @@ static void form_sigma(orc_ctx *c) {
 void orc_form_convection_les(orc_ctx *c, int istage) {
   const int mx = c->mx, my = c->my, mz = c->mz;
   if (istage == 0) form_sigma(c);
+  if (c->p.twoD == 1) zero_xi_faces(c, c->fp1, NULL);
+  else {
   #pragma omp parallel for schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 1; j < my - 1; j++)
       for (int i = 0; i < mx - 1; i++) face_conv_les(c, 0, k, j, i, &c->fp1[LI(c, k, j, i)]);
+  }
   #pragma omp parallel for schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 0; j < my - 1; j++)
@@ -432,7 +455,7 @@ void orc_calc_ucat(orc_ctx *c) {
       for (int i = 0; i < c->mx; i++) {
         const orc_q *q = &c->lQ[LI(c, k, j, i)];
         orc_v3 *u = &ucat[GI(c, k, j, i)];
-        u->x = q->rhoU / q->rho;
+        if (c->p.twoD != 1) u->x = q->rhoU / q->rho; else u->x = 0.; /* rhs.c:1349 */
         u->y = q->rhoV / q->rho;
         u->z = q->rhoW / q->rho;
       }
@@ -630,6 +653,8 @@ static void face_visc(const orc_ctx *c, int d, int k, int j, int i, orc_q *F, or
 void orc_form_viscous(orc_ctx *c) {
   const int mx = c->mx, my = c->my, mz = c->mz;
   const orc_params *P = &c->p;
+  if (P->twoD == 1) zero_xi_faces(c, c->fp1, c->al1);
+  else {
   #pragma omp parallel for schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 1; j < my - 1; j++)
@@ -637,6 +662,7 @@ void orc_form_viscous(orc_ctx *c) {
         size_t n = LI(c, k, j, i);
         face_visc(c, 0, k, j, i, &c->fp1[n], &c->al1[n]);
       }
+  }
   #pragma omp parallel for schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 0; j < my - 1; j++)
@@ -661,7 +687,10 @@ void orc_form_viscous(orc_ctx *c) {
         size_t n = LI(c, k, j, i);
         orc_q *o = &c->visc[GI(c, k, j, i)];
         o->rho = 0.;
-        o->rhoU = f1[n].rhoU - f1[n - sx].rhoU + f2[n].rhoU - f2[n - sy].rhoU + f3[n].rhoU - f3[n - sz].rhoU;
+        if (P->twoD != 1) /* rhs.c:2135-2140 */
+          o->rhoU = f1[n].rhoU - f1[n - sx].rhoU + f2[n].rhoU - f2[n - sy].rhoU + f3[n].rhoU - f3[n - sz].rhoU;
+        else
+          o->rhoU = 0.;
         o->rhoV = f1[n].rhoV - f1[n - sx].rhoV + f2[n].rhoV - f2[n - sy].rhoV + f3[n].rhoV - f3[n - sz].rhoV;
         o->rhoW = f1[n].rhoW - f1[n - sx].rhoW + f2[n].rhoW - f2[n - sy].rhoW + f3[n].rhoW - f3[n - sz].rhoW;
         o->rhoE = f1[n].rhoE - f1[n - sx].rhoE + f2[n].rhoE - f2[n - sy].rhoE + f3[n].rhoE - f3[n - sz].rhoE;

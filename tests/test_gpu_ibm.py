@@ -21,9 +21,12 @@ INT_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "
 FP_FIELDS = ("d_s", "d_i", "cs1", "cs2", "cs3", "cr1", "cr2", "cr3")
 
 
-def sphere(iso=2, les=0, level=2, jitter=1e-3):
-    return cases.sphere_channel_case(nx=16, ny=16, nz=24, Lx=2.4, Ly=2.4, Lz=3.2, zc=1.4, level=level,
-                                     jitter=jitter, isothermal=iso, les=les)
+def sphere(iso=2, les=0, level=2, jitter=1e-3, twoD=0):
+    cfg, coor, Q, V, T = cases.sphere_channel_case(nx=16, ny=16, nz=24, Lx=2.4, Ly=2.4, Lz=3.2, zc=1.4,
+                                                   level=level, jitter=jitter, isothermal=iso, les=les)
+    if twoD:
+        cfg = dict(cfg, twoD=twoD)
+    return cfg, coor, Q, V, T
 
 
 def compare_search(cfg, coor, Q, V, T):
@@ -97,3 +100,23 @@ def test_surface_forces_sphere(iso, les, steps):
     g.close()
     assert np.all(np.isfinite(fo)) and (steps == 0 or np.any(fo[0:3] != 0))
     assert np.array_equal(fg, fo), np.c_[fg, fo, fg - fo]
+
+
+@pytest.mark.parametrize("twoD", [1, 3])
+def test_steps_and_forces_sphere_two_d(twoD):
+    """-twoD with the immersed body: the interpolation normal and the surface
+    velocity drop one component (ibm_iso.c:2662-2664, 2744-2746); with 1 also
+    the flow-side rules (no xi fluxes, u = 0). Steps and the nineteen surface
+    force outputs against the oracle."""
+    cfg, coor, Q, V, T = sphere(iso=2, les=2, twoD=twoD)
+    o, g = compare_search(cfg, coor, Q, V, T)
+    o.step(2)
+    g.advance(2)
+    Qc, Pc = o.state()
+    Qg, Pg = g.state()
+    center = (1.2, 1.2, 1.4)
+    fo, fg = o.ibm_forces(center), g.ibm_forces(center)
+    g.close()
+    assert state_err(Qg, Qc) <= STATE_TOL
+    assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL
+    assert np.array_equal(fg, fo), np.c_[fg, fo]

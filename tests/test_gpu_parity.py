@@ -285,3 +285,36 @@ def test_averaging_matches_oracle(case, tmp_path):
     q_sum = cl.petsc_vec_read(str(tmp_path / ("Qfield_ave%05d_0.dat" % g.ti)), Q.size)
     assert np.array_equal(q_sum.reshape(Q.shape), g.averages()[0])
     g.close()
+
+
+@pytest.mark.parametrize("case,les", [("channel", 0), ("channel", 2), ("tgv", 2)])
+def test_steps_two_d(case, les):
+    """-twoD 1: no xi-direction fluxes, u = 0 in CalcUCat (and so in the LES
+    model's stale velocity, the statistics and the viscous stresses), no rhoU
+    residual; the channel's isothermal-2 walls take FormBCS's twoD branch
+    (rhs.c:719). Steps, Cs, nu_t and the statistics against the oracle."""
+    if case == "channel":
+        cfg, coor, Q = channel_case(bc=(7, 7, 1, 1, 45, 8), iso=2, les=les)
+    else:
+        cfg, coor, Q = cases.tgv_case(N=10, les=les)
+        Q = perturbed(Q, 9, amp=1e-2)
+    cfg = dict(cfg)
+    cfg["twoD"] = 1
+    cfg["averaging"] = 1
+    o, g = _pair(cfg, coor, Q)
+    for s in range(3):
+        o.step(1, averaging=True)
+        g.flow_step()
+        if les:
+            cso, nuo, _ = o.les_fields()
+            csg, nug, _ = g.les_fields()
+            assert np.abs(csg - cso).max() <= 1e-12 * max(np.abs(cso).max(), 1e-300), s
+            assert np.abs(nug - nuo).max() <= 1e-12 * max(np.abs(nuo).max(), 1e-300), s
+    Qc, Pc = o.state()
+    Qg, Pg = g.state()
+    for a, b in zip(g.averages(), o.averages()):
+        assert state_err(a.reshape(a.shape[:3] + (-1,)), b.reshape(b.shape[:3] + (-1,))) <= STATE_TOL
+    assert np.all(g.averages()[2][..., 0] == 0.0)  # <u u> of a zero u
+    g.close()
+    assert state_err(Qg, Qc) <= STATE_TOL
+    assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL

@@ -280,3 +280,97 @@ def test_dynamic_model_runs_from_the_first_step():
     (cs1, nut1), (cs5, nut5) = res
     assert np.array_equal(cs1, cs5) and np.array_equal(nut1, nut5)
     assert np.any(cs1 != 0) and np.any(nut1 != 0)
+
+
+# ------------------------------------------------------------------ -twoD
+
+def _yz_flow(n, x_dependent=False):
+    """Periodic Cartesian box of binary-exact spacing (identical metrics on
+    every node); density, v, w and p vary in y and z only (plus a u and an
+    x-dependence when asked)."""
+    L = 0.5 * n
+    coor, h = cases.cartesian_grid(n, L=L)
+    x, y, z = (2 * np.pi / L) * coor[..., 0], (2 * np.pi / L) * coor[..., 1], (2 * np.pi / L) * coor[..., 2]
+    rho = 1.0 + 0.1 * np.sin(y) * np.cos(2 * z)
+    v = 0.2 * np.sin(z) * np.cos(y)
+    w = -0.15 * np.cos(z) * np.sin(2 * y)
+    p = 1.0 / (1.4 * 0.3 ** 2) * (1.0 + 0.05 * np.cos(y + z))
+    u = 0 * x
+    if x_dependent:
+        u = 0.1 * np.sin(x) * np.cos(y)
+        rho = rho + 0.05 * np.cos(x)
+    Q = cases.conserved(rho, u, v, w, p)
+    return coor, Q
+
+
+@pytest.mark.parametrize("les", [0, 2])
+def test_two_d_mode_agrees_with_the_3d_residual_of_a_flow_without_x(les):
+    """A flow independent of x with u = 0: in the 3-D residual every interior
+    xi face carries the same flux, lUcat's x component is 0 / rho and the rhoU
+    residual vanishes -- what -twoD 1 imposes by skipping the xi faces and
+    zeroing rhoU and u (rhs.c:1349, 1487, 2135, 3168, 4215). The boundary xi
+    faces reconstruct first-order (or by a 2-point mean) while the interior
+    ones reproduce constants only to round-off (truncated WENO weights, the
+    4-point central flux), so the residuals agree to round-off; skipping the
+    wrong direction would differ at O(1). (Later stages are not compared: the
+    periodic seams take the stage input, which makes the 3-D flow x-dependent
+    next to the seams after the first stage.)"""
+    n = 12
+    coor, Q = _yz_flow(n)
+    r = []
+    for twoD in (0, 1):
+        cfg = dict(mx=n + 2, my=n + 2, mz=n + 2, bctype=PER, invicid=0, les=les, ren=200.0, Ma=0.3,
+                   sutherland=1, dt=2e-3, twoD=twoD)
+        o = pyoracle.Oracle(cfg, coor)
+        o.set_state(Q)
+        r.append(o.form_rhs(0)[1:-1, 1:-1, 1:-1])
+    r3, r2 = r
+    for c in range(5):
+        assert np.abs(r3[..., c] - r2[..., c]).max() <= 1e-12 * np.abs(r3).max(), c
+    assert np.all(r2[..., 1] == 0.0)
+    assert np.abs(r2[..., 3]).max() > 1e-3
+
+
+def test_two_d_mode_ignores_u_in_the_other_residuals():
+    """Two x-dependent states that differ only in u (p, rho, v, w equal): with
+    -twoD 1 the rho, rhoV and rhoW residuals do not see u (the xi faces are
+    skipped, lUcat has u = 0, the eta/zeta contravariant velocities carry no u
+    on a Cartesian grid) and agree to round-off (p is re-formed from rhoE);
+    in the 3-D run they differ at O(u)."""
+    n = 12
+    coor, Qa = _yz_flow(n, x_dependent=True)
+    x, y = (2 * np.pi / (0.5 * n)) * coor[..., 0], (2 * np.pi / (0.5 * n)) * coor[..., 1]
+    rho = Qa[..., 0]
+    u, v, w = Qa[..., 1] / rho, Qa[..., 2] / rho, Qa[..., 3] / rho
+    p = 0.4 * (Qa[..., 4] - 0.5 * rho * (u * u + v * v + w * w))
+    Qb = cases.conserved(rho, u + 0.3 * np.sin(y) * np.cos(x), v, w, p)
+    d = {}
+    for twoD in (0, 1):
+        r = []
+        for Q in (Qa, Qb):
+            cfg = dict(mx=n + 2, my=n + 2, mz=n + 2, bctype=PER, invicid=0, les=0, ren=200.0, Ma=0.3,
+                       dt=2e-3, twoD=twoD)
+            o = pyoracle.Oracle(cfg, coor)
+            o.set_state(Q)
+            r.append(o.form_rhs(0)[1:-1, 1:-1, 1:-1])
+        d[twoD] = [np.abs(r[0][..., c] - r[1][..., c]).max() / np.abs(r[0][..., c]).max() for c in (0, 2, 3)]
+    assert max(d[1]) <= 1e-12, d[1]
+    assert min(d[0]) > 1e-3, d[0]
+
+
+def test_two_d_mode_drops_the_x_momentum_residual():
+    """With an x-dependent flow the 3-D residual of rhoU is non-zero; -twoD 1
+    makes it exactly zero on every interior node while the other components
+    keep their (changed) residuals."""
+    n = 12
+    coor, Q = _yz_flow(n, x_dependent=True)
+    r = {}
+    for twoD in (0, 1):
+        cfg = dict(mx=n + 2, my=n + 2, mz=n + 2, bctype=PER, invicid=0, les=0, ren=200.0, Ma=0.3,
+                   dt=2e-3, twoD=twoD)
+        o = pyoracle.Oracle(cfg, coor)
+        o.set_state(Q)
+        r[twoD] = o.form_rhs(0)[1:-1, 1:-1, 1:-1]
+    assert np.abs(r[0][..., 1]).max() > 1e-6
+    assert np.all(r[1][..., 1] == 0.0)
+    assert np.abs(r[1][..., 0]).max() > 1e-6 and not np.array_equal(r[0][..., 0], r[1][..., 0])
