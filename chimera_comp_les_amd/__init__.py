@@ -65,7 +65,7 @@ EXPORTS = [
     "cles_batch_used", "cles_batch_set_tile", "cles_batch_advance", "cles_batch_time_step",
     "cles_batch_last_error", "cles_set_tile", "cles_batch_launch", "cles_batch_wait",
     "cles_set_state_qp", "cles_write_fields", "cles_read_fields", "cles_petsc_vec_write", "cles_petsc_vec_read",
-    "cles_get_averages", "cles_ibm_forces",
+    "cles_get_averages", "cles_ibm_forces", "cles_write_vtk",
 ]
 
 IBM_INFO_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
@@ -149,6 +149,7 @@ def load_library(path=LIB_PATH):
     L.cles_get_averages.argtypes = [vp, dp, dp, dp, dp]
     L.cles_ibm_forces.argtypes = [vp, dp, dp]
     L.cles_write_fields.argtypes = [vp, C.c_char_p, C.c_int]
+    L.cles_write_vtk.argtypes = [vp, C.c_char_p, C.c_int]
     L.cles_read_fields.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
     L.cles_petsc_vec_write.argtypes = [C.c_char_p, dp, C.c_longlong, C.c_int]
     L.cles_petsc_vec_read.argtypes = [C.c_char_p, dp, C.c_longlong]
@@ -237,6 +238,15 @@ class Solver:
         """Q_P_Binary_Output: the reference's binary field files of the current step."""
         d = None if directory is None else os.fsencode(directory)
         self._check(self.L.cles_write_fields(self.h, d, int(block)))
+
+
+    def write_vtk(self, directory=None, block=0):
+        """VTKOut (data.c:385-790): Result%2.2d_%6.6d.vts of the current state
+        in directory (default: the working directory); returns its path."""
+        d = None if directory is None else os.fsencode(directory)
+        self._check(self.L.cles_write_vtk(self.h, d, int(block)))
+        name = "Result%2.2d_%6.6d.vts" % (int(block), self.ti)
+        return name if directory is None else os.path.join(os.fspath(directory), name)
 
     def read_fields(self, ti, directory=None, block=0):
         """Q_P_Binary_Input + RestartApplication from the files of step ti."""

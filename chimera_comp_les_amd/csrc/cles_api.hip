@@ -197,6 +197,8 @@ int rk_coef(const cles_params &p, int istage, double coef[3]) {
   return rk_coefficients(p, istage, coef, &alfa);
 }
 
+int get_coor(cles_ctx *c, double *xyz) { return download(c, c->coor, xyz, 3, CLES_LAYOUT_GLOBAL); }
+
 // device bytes cles_create allocates for p (the allocation list of cles_create)
 size_t ctx_footprint(const cles_params *p) {
   const long long N = (long long)p->mx * p->my * p->mz;

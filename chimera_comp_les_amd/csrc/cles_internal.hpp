@@ -225,6 +225,8 @@ int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const
 bool supported_bc(int side, int b);
 int launch_les(cles_ctx *c);
 int launch_averaging(cles_ctx *c);
+// node coordinates, interleaved x y z in the natural order (global layout)
+int get_coor(cles_ctx *c, double *xyz);
 // batched stages of several periodic grids (serving)
 size_t stage_args_bytes();
 int stage_batch_args(cles_ctx *c, int in, int out, int istage, const double coef[3], int tile, void *dst);

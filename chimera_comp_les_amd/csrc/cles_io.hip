@@ -168,3 +168,127 @@ int cles_read_fields(cles_ctx *c, const char *dir, int ti, int block) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- VTK output
+//
+// VTKOut of the post-processor (data.c:385-790, the !Pres branch): one XML
+// StructuredGrid file Result%2.2d_%6.6d.vts (block, time step) with raw
+// appended little-endian Float64 arrays, each preceded by its byte count as
+// a 4-byte int: Position (3), nvert, p, rho, E = rhoE/rho, u = rhoU/rho ...
+// (3) and with LES Sigma (3) and nu_T.  -twoD collapses that extent to one
+// node: the fields are taken at node 1 along x (twoD 1), the coordinates at
+// node 0, and for twoD 2 / 3 every field at node 0 along y / z, as the
+// reference's loops do.  No -skip subsampling and no block rotation (theta
+// = 0 leaves the coordinates as they are).
+namespace cles {
+namespace {
+
+struct VtkArray {
+  const char *name;
+  int nc;
+};
+
+}  // namespace
+}  // namespace cles
+
+extern "C" int cles_write_vtk(cles_ctx *c, const char *dir, int block) {
+  if (!c) return CLES_EINVAL;
+  if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
+  const int MX = c->g.mx, MY = c->g.my, MZ = c->g.mz;
+  const long long N = (long long)MX * MY * MZ;
+  const int twoD = c->p.twoD;
+  const int mx = twoD == 1 ? 1 : MX, my = twoD == 2 ? 1 : MY, mz = twoD == 3 ? 1 : MZ;
+  const long long nn = (long long)mx * my * mz;
+  if (3 * nn * 8 > INT32_MAX)
+    return set_error(c, CLES_EINVAL, "VTK output: an array of %lld bytes does not fit the 4-byte size header", 3 * nn * 8);
+  std::vector<double> q((size_t)N * 5), p((size_t)N), nv((size_t)N), xyz((size_t)N * 3), nut, sig;
+  int r = cles_get_state(c, q.data(), p.data(), CLES_LAYOUT_GLOBAL);
+  if (r) return r;
+  if ((r = cles_get_nvert(c, nv.data()))) return r;
+  if ((r = get_coor(c, xyz.data()))) return r;
+  const bool les = c->p.les != 0;
+  if (les) {
+    nut.resize((size_t)N);
+    sig.resize((size_t)N * 3);
+    if ((r = cles_get_les(c, nullptr, nut.data(), sig.data()))) return r;
+  }
+  std::vector<VtkArray> arrs = {{"nvert", 1}, {"p", 1}, {"rho", 1}, {"E", 1}, {"u", 3}};
+  if (les) {
+    arrs.push_back({"Sigma", 3});
+    arrs.push_back({"nu_T", 1});
+  }
+  char path[1024];
+  if (dir && dir[0])
+    snprintf(path, sizeof(path), "%s/Result%2.2d_%6.6d.vts", dir, block, c->ti);
+  else
+    snprintf(path, sizeof(path), "Result%2.2d_%6.6d.vts", block, c->ti);
+  FILE *f = fopen(path, "wb");
+  if (!f) return set_error(c, CLES_EINVAL, "cannot create %s: %s", path, strerror(errno));
+  long long off = 0;
+  fprintf(f, "<?xml version=\"1.0\"?>\n");
+  fprintf(f, "<VTKFile type=\"StructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n");
+  fprintf(f, "  <StructuredGrid WholeExtent=\"%d %d %d %d %d %d\">\n", 0, mx - 1, 0, my - 1, 0, mz - 1);
+  fprintf(f, "    <Piece Extent=\"%d %d %d %d %d %d\">\n", 0, mx - 1, 0, my - 1, 0, mz - 1);
+  fprintf(f, "      <Points>\n");
+  fprintf(f, "        <DataArray type=\"Float64\" Name=\"Position\" NumberOfComponents=\"3\" format=\"appended\" offset=\"%lld\" />\n", off);
+  off += 3 * nn * 8 + 4;
+  fprintf(f, "      </Points>\n");
+  fprintf(f, "      <PointData Scalars=\"ScalarPointData\">\n");
+  for (const VtkArray &a : arrs) {
+    fprintf(f, "        <DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"%d\" format=\"appended\" offset=\"%lld\" />\n",
+            a.name, a.nc, off);
+    off += a.nc * nn * 8 + 4;
+  }
+  fprintf(f, "      </PointData>\n");
+  fprintf(f, "    </Piece>\n");
+  fprintf(f, "  </StructuredGrid>\n");
+  fprintf(f, "  <AppendedData encoding=\"raw\">\n");
+  fprintf(f, "_");
+  std::vector<double> out((size_t)nn * 3);
+  bool ok = true;
+  auto put = [&](int nc) {
+    const int bytes = (int)(nc * nn * 8);
+    ok = ok && fwrite(&bytes, sizeof(int), 1, f) == 1 && fwrite(out.data(), 8, (size_t)(nc * nn), f) == (size_t)(nc * nn);
+  };
+  // source node of output node (k, j, i): the fields read x node 1 under
+  // twoD 1 (Sigma: i + 1 = 1), the coordinates x node i
+  auto src = [&](int k, int j, int i, bool field) {
+    const int ii = field && twoD == 1 ? 1 : i;
+    return ((long long)k * MY + j) * MX + ii;
+  };
+  for (int k = 0; k < mz; k++)  // Position
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) {
+        const long long o = ((long long)k * my + j) * mx + i, s = src(k, j, i, false);
+        for (int d = 0; d < 3; d++) out[3 * o + d] = xyz[3 * s + d];
+      }
+  put(3);
+  for (size_t kind = 0; kind < arrs.size(); kind++) {  // nvert p rho E u [Sigma nu_T]
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+          const long long o = ((long long)k * my + j) * mx + i, s = src(k, j, i, true);
+          const double *qs = &q[5 * s];
+          switch (kind) {
+            case 0: out[o] = nv[s]; break;
+            case 1: out[o] = p[s]; break;
+            case 2: out[o] = qs[0]; break;
+            case 3: out[o] = qs[4] / qs[0]; break;
+            case 4:
+              out[3 * o] = qs[1] / qs[0];
+              out[3 * o + 1] = qs[2] / qs[0];
+              out[3 * o + 2] = qs[3] / qs[0];
+              break;
+            case 5:
+              for (int d = 0; d < 3; d++) out[3 * o + d] = sig[3 * s + d];
+              break;
+            default: out[o] = nut[s];
+          }
+        }
+    put(arrs[kind].nc);
+  }
+  fprintf(f, "\n </AppendedData>\n");
+  fprintf(f, "</VTKFile>\n");
+  ok = (fclose(f) == 0) && ok;
+  return ok ? CLES_OK : set_error(c, CLES_EINVAL, "write error on %s", path);
+}

@@ -163,6 +163,12 @@ int cles_get_state(cles_ctx *c, double *Q, double *P, int layout);
  * Qold = Q, P as read and the time step to ti.  An immersed body is
  * classified again afterwards (cles_ibm_search), as main.c:870-877 does. */
 int cles_write_fields(cles_ctx *c, const char *dir, int block);
+/* VTKOut of the post-processor (data.c:385-790): dir/Result%2.2d_%6.6d.vts
+ * (block, time step), an XML StructuredGrid with raw appended little-endian
+ * Float64 arrays Position, nvert, p, rho, E, u and with LES Sigma, nu_T, each
+ * after a 4-byte byte count (so each array stays under 2 GiB); -twoD takes
+ * one x/y/z node as the reference's writer does. */
+int cles_write_vtk(cles_ctx *c, const char *dir, int block);
 int cles_read_fields(cles_ctx *c, const char *dir, int ti, int block);
 /* One PETSc binary Vec file: class id 1211214 and length as big-endian
  * int32, then n big-endian doubles; block_size > 1 also writes <path>.info

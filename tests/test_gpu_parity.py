@@ -318,3 +318,79 @@ def test_steps_two_d(case, les):
     g.close()
     assert state_err(Qg, Qc) <= STATE_TOL
     assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL
+
+
+def _read_vts(path):
+    """Minimal reader of an XML StructuredGrid with raw appended Float64
+    arrays: the extent, and each array's name, components and values (the
+    4-byte byte count before each block checked against the offsets)."""
+    import re
+    data = open(path, "rb").read()
+    mark = b'  <AppendedData encoding="raw">\n_'
+    head_end = data.index(mark)
+    header = data[:head_end].decode()
+    base = head_end + len(mark)
+    ext = [int(v) for v in re.search(r'WholeExtent="([-\d ]+)"', header).group(1).split()]
+    arrays = re.findall(r'<DataArray type="Float64" Name="(\w+)" NumberOfComponents="(\d+)" '
+                        r'format="appended" offset="(\d+)" />', header)
+    out, pos = {}, base
+    for name, nc, off in arrays:
+        assert base + int(off) == pos, name
+        nbytes = int(np.frombuffer(data, dtype="<i4", count=1, offset=pos)[0])
+        out[name] = (int(nc), np.frombuffer(data, dtype="<f8", count=nbytes // 8, offset=pos + 4))
+        pos += 4 + nbytes
+    assert data[pos:] == b"\n </AppendedData>\n</VTKFile>\n"
+    return header, ext, out
+
+
+@pytest.mark.parametrize("case", ["tgv_les", "channel_two_d"])
+def test_vtk_output(case, tmp_path):
+    """VTKOut (data.c:385-790): the StructuredGrid header, the appended
+    arrays in the reference's order with consistent offsets and byte counts,
+    and their values: coordinates, nvert, p, rho, E = rhoE/rho, u = rhoU/rho
+    ..., and with LES Sigma and nu_T; -twoD 1 keeps one x node (fields from
+    node 1, coordinates from node 0)."""
+    if case == "tgv_les":
+        cfg, coor, Q0 = cases.tgv_case(N=10)
+        Q0 = perturbed(Q0, 4, amp=1e-2)
+    else:
+        cfg, coor, Q0 = channel_case(bc=(7, 7, 1, 1, 45, 8), iso=0, les=0)
+        cfg = dict(cfg, twoD=1)
+    g = cl.Solver(cfg, coor)
+    g.set_state(Q0)
+    g.advance(2)
+    path = g.write_vtk(str(tmp_path), block=3)
+    assert os.path.basename(path) == "Result03_%06d.vts" % g.ti
+    Q, P = g.state()
+    nv = g.nvert()
+    les = cfg.get("les", 0) != 0
+    if les:
+        _, nut, sig = g.les_fields()
+    g.close()
+    header, ext, arr = _read_vts(path)
+    mz, my, mx = Q.shape[:3]
+    if case == "channel_two_d":
+        sl, xs = (slice(None), slice(None), slice(1, 2)), (slice(None), slice(None), slice(0, 1))
+        mx = 1
+    else:
+        sl = xs = (slice(None),) * 3
+    assert ext == [0, mx - 1, 0, my - 1, 0, mz - 1]
+    assert header.startswith('<?xml version="1.0"?>\n<VTKFile type="StructuredGrid" version="0.1" '
+                             'byte_order="LittleEndian">\n')
+    names = ["nvert", "p", "rho", "E", "u"] + (["Sigma", "nu_T"] if les else [])
+    assert list(arr) == ["Position"] + names
+    want = {
+        "Position": coor[xs],
+        "nvert": nv[sl],
+        "p": P[sl],
+        "rho": Q[sl][..., 0],
+        "E": Q[sl][..., 4] / Q[sl][..., 0],
+        "u": Q[sl][..., 1:4] / Q[sl][..., 0:1],
+    }
+    if les:
+        want["Sigma"] = sig[sl]
+        want["nu_T"] = nut[sl]
+    for name, (nc, vals) in arr.items():
+        w = np.ascontiguousarray(want[name]).reshape(-1)
+        assert nc * mx * my * mz == vals.size == w.size, name
+        assert np.array_equal(vals, w), name
