@@ -286,12 +286,21 @@ extern "C" int cles_ibm_forces(cles_ctx *c, const double center[3], double out[C
   if (!b) return set_error(c, CLES_ESTATE, "no immersed body (cles_ibm_search)");
   for (int q = 0; q < CLES_FORCE_N; q++) out[q] = 0.;
   if (b->n_ib == 0) return CLES_OK;
-  double *rec = nullptr, *dout = nullptr;
-  if (hipMalloc((void **)&rec, (size_t)b->n_ib * kRec * sizeof(double)) != hipSuccess ||
-      hipMalloc((void **)&dout, CLES_FORCE_N * sizeof(double)) != hipSuccess) {
-    if (rec) (void)hipFree(rec);
-    return set_error(c, CLES_ENOMEM, "force work buffers");
+  // work buffers of the body's list, made at the first call and released
+  // with the body (a new cles_ibm_search makes a new list)
+  if (!b->frec) {
+    if (hipMalloc((void **)&b->frec, (size_t)b->n_ib * kRec * sizeof(double)) != hipSuccess) {
+      b->frec = nullptr;
+      return set_error(c, CLES_ENOMEM, "force work buffers");
+    }
   }
+  if (!b->fout) {
+    if (hipMalloc((void **)&b->fout, CLES_FORCE_N * sizeof(double)) != hipSuccess) {
+      b->fout = nullptr;
+      return set_error(c, CLES_ENOMEM, "force work buffers");
+    }
+  }
+  double *rec = b->frec, *dout = b->fout;
   ForceArgs a;
   a.g = c->g;
   a.n_ib = b->n_ib;
@@ -315,8 +324,6 @@ extern "C" int cles_ibm_forces(cles_ctx *c, const double center[3], double out[C
   }
   if (e == hipSuccess) e = hipMemcpyAsync(out, dout, CLES_FORCE_N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(rec);
-  (void)hipFree(dout);
   if (e != hipSuccess) return set_error(c, CLES_EDEVICE, "surface forces: %s", hipGetErrorString(e));
   return CLES_OK;
 }

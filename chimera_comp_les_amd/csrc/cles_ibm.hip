@@ -799,7 +799,7 @@ void ibm_free(cles_ctx *c) {
   cles_ibm_dev *b = c->ibm;
   if (!b) return;
   void *bufs[] = {b->vx, b->vy, b->vz, b->tv, b->nf, b->qv, b->rad, b->u, b->uold, b->node, b->cell, b->imode,
-                  b->don, b->dn, b->info, b->nrm, b->w, b->carry};
+                  b->don, b->dn, b->info, b->nrm, b->w, b->carry, b->frec, b->fout};
   for (void *q : bufs)
     if (q) (void)hipFree(q);
   delete b;

@@ -169,6 +169,7 @@ struct cles_ibm_dev {
   double *nrm;           // 3 planes: wall normal used by the interpolation
   double *w;             // per-sweep work planes
   double *carry;         // Pa carried from one sweep to the next
+  double *frec, *fout;   // surface-force records (n_ib of them) and outputs, made at the first call
 };
 
 struct cles_ctx {
@@ -183,7 +184,7 @@ struct cles_ctx {
   double *fil2;             // (pow(1/aj, 1/3))^2: squared LES filter width per node
   double *tfil2;            // squared test-filter width (2 or 5^(1/3) filter widths)
   double ref_rho, ref_p;    // FP32-bulk reference state (node (1,1,1) of the uploaded state)
-  int tile;                 // stage-kernel tile shape (0: 32x4x2, 1: 16x4x4, 2: 16x8x2, 3: 64x2x2)
+  int tile;                 // stage-kernel tile shape 0..7 (cles_set_tile), default 7: 8x4x3 in 128 threads
   int has_ib;               // any nvert != 0
   int immersed;             // a surface was classified: RK steps end with the interpolation
   cles_ibm_dev *ibm;

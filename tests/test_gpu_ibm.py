@@ -97,8 +97,13 @@ def test_surface_forces_sphere(iso, les, steps):
     center = (1.2, 1.2, 1.4)
     fo = o.ibm_forces(center)
     fg = g.ibm_forces(center)
-    g.close()
     assert np.all(np.isfinite(fo)) and (steps == 0 or np.any(fo[0:3] != 0))
+    assert np.array_equal(fg, fo), np.c_[fg, fo, fg - fo]
+    # once per step, as Struc_Solver calls it: the work buffers are reused
+    o.step(1)
+    g.advance(1)
+    fo, fg = o.ibm_forces(center), g.ibm_forces(center)
+    g.close()
     assert np.array_equal(fg, fo), np.c_[fg, fo, fg - fo]
 
 
