@@ -170,6 +170,9 @@ __device__ __forceinline__ long long seam_image(const Grid &g, int k, int j, int
 // quotient rounded once).
 constexpr int FTX = 8, FTY = 8, FTZ = 4;
 constexpr double FILTER_INV_WSUM = 0.125;
+#ifndef CLES_FILTER_XZY
+#define CLES_FILTER_XZY 1
+#endif
 constexpr int FBX = FTX + 2, FBY = FTY + 2, FBZ = FTZ + 2, FNB = FBX * FBY * FBZ;
 static_assert(FTX * FTY * FTZ == 256, "one node per thread");
 
@@ -183,7 +186,13 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a, int ntx, int
   const long long np = g.np;
   const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);  // contiguous tiles per XCD
   if (tile >= ntiles) return;
+#if CLES_FILTER_XZY
+  // x fastest, then z, then y (the tiles in flight share their z halos)
+  const int ntz = ntiles / (ntx * nty);
+  const int i0 = 1 + (tile % ntx) * FTX, k0 = 1 + ((tile / ntx) % ntz) * FTZ, j0 = 1 + (tile / (ntx * ntz)) * FTY;
+#else
   const int i0 = 1 + (tile % ntx) * FTX, j0 = 1 + ((tile / ntx) % nty) * FTY, k0 = 1 + (tile / (ntx * nty)) * FTZ;
+#endif
   const int t = threadIdx.x;
   const int lx = t % FTX, ly = (t / FTX) % FTY, lz = t / (FTX * FTY);
   const int i = i0 + lx, j = j0 + ly, k = k0 + lz;

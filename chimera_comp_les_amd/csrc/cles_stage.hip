@@ -69,6 +69,10 @@ constexpr int NT = 256;    // threads per workgroup of the larger tile shapes
 #ifndef CLES_STAGE_PF_LESE
 #define CLES_STAGE_PF_LESE 1
 #endif
+// order of the tiles along an XCD's range: 1 = x, z, y; 0 = x, y, z
+#ifndef CLES_TILE_XZY
+#define CLES_TILE_XZY 1
+#endif
 constexpr int NSLOT = 64;  // spread of the inf-norm atomics
 
 // Tile of interior nodes owned by one workgroup, with the face counts of its
@@ -798,7 +802,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   __shared__ double Ss[STG ? NCH * NSM : 1];
   const Grid &g = a.g;
   if (tile >= a.ntx * a.nty * a.ntz) return;
+#if CLES_TILE_XZY
+  // x fastest, then z, then y: the tiles in flight on an XCD share their z
+  // halos (the deepest: a 3-node tile with +-1 and +-2 stencils) in its L2
+  const int tix = tile % a.ntx, tiz = (tile / a.ntx) % a.ntz, tiy = tile / (a.ntx * a.ntz);
+#else
   const int tix = tile % a.ntx, tiy = (tile / a.ntx) % a.nty, tiz = tile / (a.ntx * a.nty);
+#endif
   const int i0 = 1 + tix * TX, j0 = 1 + tiy * TY, k0 = 1 + tiz * TZ;
   const int t = threadIdx.x;
   const long long np = g.np;
