@@ -1,8 +1,8 @@
 """ctypes binding of the CPU restatement (oracle/liboracle.so).
 
 Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and
-by bench.py's cpu_baseline leg, always as the checker, never as the thing
-measured or shipped.
+by bench.py's cpu_baseline leg, always as the reference the results are
+compared with, never as the thing measured or shipped.
 """
 import ctypes as C
 import os

@@ -1,6 +1,6 @@
 """Shared helpers of the parity tests: small cases and tolerance checks.
 
-Tolerances (FP64 path vs the CPU restatement, BASELINE bar 1e-12):
+Tolerances (FP64 path vs the CPU restatement, a 1e-12 relative bar):
   * right-hand side: |gpu - cpu| <= 1e-12 * ||cpu_c||_inf per component c
     (the RHS is a difference of face fluxes, so it is normalised by the
     component's scale, not by its own value at cells where it cancels);
