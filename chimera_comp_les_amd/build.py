@@ -1,4 +1,4 @@
-"""Build the gfx950 library (libcles.so) and the CPU checker in place.
+"""Build the gfx950 library (libcles.so) and the CPU restatement in place.
 
     python -m chimera_comp_les_amd.build          # both
     python -m chimera_comp_les_amd.build --lib    # HIP library only

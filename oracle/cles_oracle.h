@@ -1,7 +1,7 @@
 /*
  * cles_oracle.h -- CPU restatement of the Chimera_Comp_LES right-hand-side and
- * time-advance path, used ONLY as the correctness checker and as the CPU
- * baseline timer.  It is test infrastructure: the product library
+ * time-advance path, used ONLY as the parity reference of the tests and as
+ * the CPU baseline timer.  It is test infrastructure: the product library
  * (chimera_comp_les_amd/csrc) never includes, links or calls it.
  *
  * Layout follows the reference's PETSc DMDA single-rank arrays:
