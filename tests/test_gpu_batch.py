@@ -132,3 +132,33 @@ def test_large_viscous_member_equals_solo(prec):
         b.close()
         s.close()
         assert np.array_equal(Q, ref[0]) and np.array_equal(P, ref[1]), tile
+
+
+def test_two_d_members_equal_solo():
+    """-twoD 1 members (no xi fluxes, u = 0, no rhoU residual) batched with
+    3-D members: every member's result is its solo result bit for bit."""
+    reqs = []
+    for kind, N, Ma, Re, prec, twoD in (("tgv", 20, 0.2, 500.0, cl.PREC_FP64, 1),
+                                         ("tgv", 16, 0.3, 800.0, cl.PREC_FP32BULK, 1),
+                                         ("vortex", 16, 0.4, None, cl.PREC_FP64, 0)):
+        cfg, coor, Q, prec = request(kind, N, Ma, Re, prec)
+        reqs.append((dict(cfg, twoD=twoD), coor, Q, prec))
+    refs = [solo(r, 6) for r in reqs]
+    b = cl.Batch(1 << 40)
+    ms = [member(r) for r in reqs]
+    for s in ms:
+        assert b.add(s)
+    b.advance(6)
+    outs = [s.state() for s in ms]
+    b.close()
+    for s in ms:
+        s.close()
+    for (Q, P), (Qr, Pr) in zip(outs, refs):
+        assert np.array_equal(Q, Qr) and np.array_equal(P, Pr)
+    # no rhoU residual: a periodic -twoD 1 member keeps its initial rhoU up to
+    # the rounding of the RK combinations (0.75 q + 0.25 q ...), while rho moves
+    for k in (0, 1):
+        Q0 = reqs[k][2]
+        du = np.abs(outs[k][0][..., 1] - Q0[..., 1]).max()
+        assert du <= 1e-14 * np.abs(Q0[..., 1]).max(), du
+        assert np.abs(outs[k][0][..., 0] - Q0[..., 0]).max() > 1e-8
