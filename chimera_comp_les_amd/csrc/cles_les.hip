@@ -13,8 +13,9 @@
 // field is ever stored.
 //
 // Kernels: gradients -> 27-point test filter (L_ij M_ij contraction) ->
-// 27-point <LM>/<MM> + clipping -> volume-weighted 27-point smoothing ->
-// nu_t.  Each is one thread per node with i-contiguous loads.
+// 27-point <LM>/<MM> + clipping -> volume-weighted 27-point smoothing and
+// nu_t (a separate nu_t kernel on the paths that run no test filter).  Each
+// is one thread per node with i-contiguous loads.
 #include "cles_internal.hpp"
 
 namespace cles {
@@ -440,7 +441,9 @@ __device__ __forceinline__ double lcs_at(const LesArgs &a, int k, int j, int i) 
   return a.cs[s >= 0 ? s : nid(a.g, k, j, i)];
 }
 
-// volume-weighted smoothing of Cs over 27 nodes (les.c:1258-1337)
+// volume-weighted smoothing of Cs over 27 nodes (les.c:1258-1337), then
+// nu_t = Cs Delta^2 |S| of the same node (les.c:1536-1573) from the smoothed
+// Cs and the |S| the test filter formed
 __global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs a, int ntx, int nty, int ntiles, int per_xcd) {
   __shared__ double B[2][MNB];
   const Grid &g = a.g;
@@ -456,23 +459,28 @@ __global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs a, int ntx, int
   __syncthreads();
   if (!T.valid) return;
   const long long n = nid(g, T.k, T.j, T.i);
-  if (!interior(g, T.k, T.j, T.i) || !(nvat(a, n) < 0.1) || a.les == 3) {
-    a.cs_out[n] = a.cs[n];
-    return;
+  const bool in = interior(g, T.k, T.j, T.i);
+  const double nv = nvat(a, n);
+  double cs;
+  if (!in || !(nv < 0.1) || a.les == 3) {
+    cs = a.cs[n];
+  } else {
+    double vs = 0.0, ws = 0.0;
+#pragma unroll
+    for (int c = -1; c <= 1; c++)
+#pragma unroll
+      for (int b = -1; b <= 1; b++)
+#pragma unroll
+        for (int aa = -1; aa <= 1; aa++) {
+          const int o = T.c0 + (c * MBY + b) * MBX + aa;
+          const double w = B[0][o];
+          vs += w * B[1][o];
+          ws += w;
+        }
+    cs = vs / ws;
   }
-  double vs = 0.0, ws = 0.0;
-#pragma unroll
-  for (int c = -1; c <= 1; c++)
-#pragma unroll
-    for (int b = -1; b <= 1; b++)
-#pragma unroll
-      for (int aa = -1; aa <= 1; aa++) {
-        const int o = T.c0 + (c * MBY + b) * MBX + aa;
-        const double w = B[0][o];
-        vs += w * B[1][o];
-        ws += w;
-      }
-  a.cs_out[n] = vs / ws;
+  a.cs_out[n] = cs;
+  a.nut[n] = !in || nv > 0.1 ? 0.0 : cs * a.fil2[n] * a.sabs[n];
 }
 
 // nu_t = Cs Delta^2 |S| (les.c:1536-1573)
@@ -502,19 +510,6 @@ __global__ __launch_bounds__(256) void les_nut_kernel(LesArgs a, int per_xcd) {
   grad9(a, n, up_i, um_i, up_j, um_j, up_k, um_k, gr);
   const double Sabs = sabs_of(gr);
   a.nut[n] = a.cs[n] * a.fil2[n] * Sabs;
-}
-
-// nu_t from the |S| the test filter formed (same product as les_nut_kernel)
-__global__ __launch_bounds__(256) void les_nut_stored_kernel(LesArgs a, int per_xcd) {
-  const Grid &g = a.g;
-  const long long n = xcd_node(per_xcd);
-  if (n >= (long long)g.mx * g.my * g.mz) return;
-  const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
-  if (!interior(g, k, j, i) || nvat(a, n) > 0.1) {
-    a.nut[n] = 0.0;
-    return;
-  }
-  a.nut[n] = a.cs[n] * a.fil2[n] * a.sabs[n];
 }
 
 __global__ __launch_bounds__(256) void fill_kernel(double *x, long long n, double v) {
@@ -583,11 +578,10 @@ int launch_les(cles_ctx *c) {
     a.cs = c->cs;
     c->ucat_les_mixed = 1;
   }
-  if (a.ucat_updated)
-    hipLaunchKernelGGL(les_nut_stored_kernel, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
-  else
+  if (!a.ucat_updated) {  // otherwise les_smooth_kernel has formed nu_t
     hipLaunchKernelGGL(les_nut_kernel, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
-  CLES_HIP(c, hipGetLastError());
+    CLES_HIP(c, hipGetLastError());
+  }
   return CLES_OK;
 }
 
