@@ -344,10 +344,15 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs a, int ntx, int
 }
 
 // The 27-point means of les_cs_kernel and les_smooth_kernel run over tiles of
-// every node (boundary layers included): an 8x8x4 block of nodes and its
-// one-node halo staged in LDS, blocks placed contiguously per XCD.  Only
+// every node (boundary layers included): a 16x4x4 block of nodes and its
+// one-node halo staged in LDS, blocks placed contiguously per XCD (16-node
+// rows: an 18-node halo row spans fewer partly used 128 B lines than 8x8x4).  Only
 // interior nodes read the halo, and their neighbours always lie in the grid.
-constexpr int MTX = 8, MTY = 8, MTZ = 4;
+#ifndef CLES_MEAN_TILE
+#define CLES_MEAN_TILE 16, 4, 4
+#endif
+constexpr int MT_[3] = {CLES_MEAN_TILE};
+constexpr int MTX = MT_[0], MTY = MT_[1], MTZ = MT_[2];
 constexpr int MBX = MTX + 2, MBY = MTY + 2, MBZ = MTZ + 2, MNB = MBX * MBY * MBZ;
 static_assert(MTX * MTY * MTZ == 256, "one node per thread");
 
