@@ -40,7 +40,6 @@ __device__ __forceinline__ void bnd_decode(const Grid &g, long long b, int &k, i
   const long long mx = g.mx, my = g.my, mz = g.mz;
   const long long pk = mx * my;              // k = 0 and k = mz-1 planes
   const long long pj = mx * (mz - 2);        // j = 0 and j = my-1, 1<=k<=mz-2
-  const long long pi = (my - 2) * (mz - 2);  // i = 0 and i = mx-1, interior j,k
   if (b < 2 * pk) {
     k = b < pk ? 0 : (int)(mz - 1);
     long long r = b % pk;
@@ -57,8 +56,11 @@ __device__ __forceinline__ void bnd_decode(const Grid &g, long long b, int &k, i
     return;
   }
   b -= 2 * pj;
-  i = b < pi ? 0 : (int)(mx - 1);
-  long long r = b % pi;
+  // i = 0 and i = mx-1 with interior j, k: (my-2)(mz-2) nodes each.  The two
+  // nodes of one (j, k) go to neighbouring threads:
+  // each shares its 128 B line with the other's seam source (i = 1, mx-2)
+  i = (b & 1) ? (int)(mx - 1) : 0;
+  const long long r = b >> 1;
   k = 1 + (int)(r / (my - 2));
   j = 1 + (int)(r % (my - 2));
 }
