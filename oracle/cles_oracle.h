@@ -108,6 +108,8 @@ void orc_flow_step(orc_ctx *c);   /* main.c:900-974 one step */
 void orc_do_averaging(orc_ctx *c);
 void orc_get_averages(const orc_ctx *c, double *q_sum, double *p_sum, double *u2_sum, double *ucross_sum);
 void orc_norms(const orc_ctx *c, double out[2]);       /* solvers.c:251-252 */
+/* G2L of Q and EqOfState, as RungeKutta runs them after FormBCS */
+void orc_bcs_tail(orc_ctx *c);
 
 /* Immersed body.  Classification + interpolation list
  * (ibm_search_advanced, ibm_iso.c:87-795) for one triangulated surface;

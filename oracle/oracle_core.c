@@ -295,16 +295,18 @@ void orc_get_les(const orc_ctx *c, double *cs, double *nu_t, double *sigma) {
   if (sigma) memcpy(sigma, c->Sigma, c->nG * sizeof(orc_v3));
 }
 
-/* VecNorm(NORM_INFINITY) of Rhs and Q, solvers.c:251-252 */
+/* VecNorm(NORM_INFINITY) of Rhs and Q as RungeKutta takes them after the
+ * last formRHS (solvers.c:251-252): the Rhs of that stage and its input Q. */
 void orc_norms(const orc_ctx *c, double out[2]) {
-  double a = 0, b = 0;
-  const double *r = (const double *)c->Rhs, *q = (const double *)c->Q;
-  for (size_t n = 0; n < 5 * c->nG; n++) {
-    double x = fabs(r[n]), y = fabs(q[n]);
-    if (x > a) a = x;
-    if (y > b) b = y;
-  }
-  out[0] = a; out[1] = b;
+  out[0] = c->nrm[0];
+  out[1] = c->nrm[1];
+}
+
+/* What RungeKutta runs after FormBCS: G2L of Q, then EqOfState
+ * (solvers.c:276-279). */
+void orc_bcs_tail(orc_ctx *c) {
+  orc_g2l(c, (double *)c->Q, (double *)c->lQ, 5);
+  orc_eq_of_state(c);
 }
 
 /* ------------------------------------------------------------------------

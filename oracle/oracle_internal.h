@@ -26,6 +26,7 @@ struct orc_ctx {
   double *P_sum;
   orc_v3 *U2_sum, *Ucross_sum;
   int ti;
+  double nrm[2];         /* |Rhs|inf, |Q|inf of the last formRHS and its input */
   /* per-call temporaries the reference obtains with VecDuplicate */
   orc_q *fp1, *fp2, *fp3, *conv, *visc;
   orc_v3 *al1, *al2, *al3;

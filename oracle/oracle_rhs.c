@@ -742,6 +742,19 @@ void orc_form_rhs(orc_ctx *c, int istage, double *rhs_out) {
           o->rho = o->rhoU = o->rhoV = o->rhoW = o->rhoE = 0.;
         }
       }
+  /* VecNorm(Rhs) and VecNorm(Q) right after formRHS, before the stage update
+   * (solvers.c:249-252): |Q| is the norm of the stage input on every owned node */
+  {
+    double a = 0, b = 0;
+    const double *q = (const double *)c->Q;
+    for (size_t n = 0; n < 5 * c->nG; n++) {
+      const double x = fabs(r[n]), y = fabs(q[n]);
+      if (x > a) a = x;
+      if (y > b) b = y;
+    }
+    c->nrm[0] = a;
+    c->nrm[1] = b;
+  }
   if (rhs_out) memcpy(rhs_out, c->Rhs, c->nG * sizeof(orc_q));
 }
 

@@ -66,6 +66,7 @@ def lib():
         L.orc_eq_of_state.argtypes = [C.c_void_p]
         L.orc_form_rhs.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_form_bcs.argtypes = [C.c_void_p, C.c_double]
+        L.orc_bcs_tail.argtypes = [C.c_void_p]
         L.orc_runge_kutta.argtypes = [C.c_void_p]
         L.orc_les_update.argtypes = [C.c_void_p]
         L.orc_flow_step.argtypes = [C.c_void_p]
@@ -203,6 +204,12 @@ class Oracle:
 
     def runge_kutta(self):
         self.L.orc_runge_kutta(self.h)
+
+    def form_bcs(self, alfa):
+        """FormBCS(user, alfa) on the current Q (= lQ = Qold after set_state),
+        then the G2L and EqOfState that RungeKutta runs after it."""
+        self.L.orc_form_bcs(self.h, float(alfa))
+        self.L.orc_bcs_tail(self.h)
 
     def les_update(self):
         self.L.orc_les_update(self.h)
