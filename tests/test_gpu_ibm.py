@@ -2,10 +2,10 @@
 time steps with the interpolation, HIP path through the C ABI against the CPU
 restatement.
 
-Classification (Nvert) and every integer of the stencils (node, triangle,
-ICP mode, donor indices) must be identical; the floating-point stencil data
-(nearest point, distances, weights) within 1e-13 relative; the state after
-time steps within the FP64 bar of parity_util.
+Classification (Nvert) and the stencils -- every integer (node, triangle,
+ICP mode, donor indices) and every floating-point field (nearest point,
+distances, weights) -- must be bit-identical; the state after time steps
+within the FP64 bar of parity_util.
 """
 import numpy as np
 import pytest
@@ -42,8 +42,8 @@ def compare_search(cfg, coor, Q, V, T):
     for f in INT_FIELDS:
         assert np.array_equal(Lo[f], Lg[f]), f
     for f in FP_FIELDS:
-        np.testing.assert_allclose(Lg[f], Lo[f], rtol=1e-13, atol=1e-15, err_msg=f)
-    np.testing.assert_allclose(Lg["pmin"], Lo["pmin"], rtol=1e-13, atol=1e-15)
+        assert np.array_equal(Lg[f], Lo[f]), f
+    assert np.array_equal(Lg["pmin"], Lo["pmin"])
     return o, g
 
 

@@ -150,15 +150,72 @@ def test_steps_tgv_les_modes(les):
     assert state_err(Qg, Qc) <= STATE_TOL
 
 
-def test_norms_match():
-    cfg, coor, Q = small_vortex(N=10)
+@pytest.mark.parametrize("case", ["periodic", "channel"])
+def test_norms_match(case):
+    """|Rhs|inf and |Q|inf as RungeKutta prints them (solvers.c:251-252): the
+    last stage's residual and its input state over every owned node.  The
+    fully periodic grid takes the boundary share of |Q| in the fused
+    seam/EOS pass, the channel in the boundary update pass."""
+    if case == "periodic":
+        cfg, coor, Q = small_vortex(N=10)
+    else:
+        cfg, coor, Q = channel_case(bc=(7, 7, 1, 1, 45, 8), iso=2)
     o, g = _pair(cfg, coor, Q)
-    o.step(1)
-    g.advance(1)
+    o.step(2)
+    g.advance(2)
     no, ng = o.norms(), g.norms()
     g.close()
-    # the oracle's VecNorm(Q) is taken before the last stage's update, like the GPU's
+    assert no[0] > 0 and no[1] > 0
     assert abs(ng[0] - no[0]) <= 1e-12 * no[0]
+    assert abs(ng[1] - no[1]) <= 1e-12 * no[1]
+
+
+@pytest.mark.parametrize("fluxorder", [3, 2])
+def test_hybrid_sigma_blend(fluxorder):
+    """-s_x/-s_y/-s_z below 1: the LES convective flux is the blend
+    (1 - sigma) F_upwind + sigma F_central (rhs.c:3698-3702) with the upwind
+    flux of WENO or MUSCL + Rusanov, and inside the domain sigma_y = sigma_z =
+    min(s_y, s_z) (rhs.c:3436-3437); the LES stresses and nu_t carry sigma
+    too (rhs.c:1671-1673)."""
+    cfg, coor, Q = cases.tgv_case(N=12, fluxorder=fluxorder)
+    cfg.update(s_x=0.5, s_y=0.3, s_z=0.7)
+    Q = perturbed(Q, 9, amp=1e-2)
+    o, g = _pair(cfg, coor, Q)
+    e = rhs_err(g.form_rhs(0), o.form_rhs(0))
+    assert e <= RHS_TOL, e
+    _, _, sg = o.les_fields()
+    assert np.allclose(sg[1:-1, 1:-1, 1:-1], (0.5, 0.3, 0.3)) and np.allclose(sg[0, 0, 0], (0.5, 0.3, 0.7))
+    for s in range(3):
+        o.step(1)
+        g.flow_step()
+    Qc, _ = o.state()
+    Qg, _ = g.state()
+    g.close()
+    assert state_err(Qg, Qc) <= STATE_TOL
+
+
+def test_steps_tgv_les_testfilter_ik():
+    """-testfilter_ik: test-filter width 5^(1/3) Delta instead of 2 Delta."""
+    cfg, coor, Q = cases.tgv_case(N=12)
+    cfg["testfilter_ik"] = 1
+    Q = perturbed(Q, 11, amp=1e-2)
+    o, g = _pair(cfg, coor, Q)
+    ref = pyoracle.Oracle(dict(cfg, testfilter_ik=0), coor)
+    ref.set_state(Q)
+    for s in range(3):
+        o.step(1)
+        ref.step(1)
+        g.flow_step()
+        cso, nuo, _ = o.les_fields()
+        csg, nug, _ = g.les_fields()
+        assert np.abs(csg - cso).max() <= 1e-12 * max(np.abs(cso).max(), 1e-300), s
+        assert np.abs(nug - nuo).max() <= 1e-12 * max(np.abs(nuo).max(), 1e-300), s
+    # the option changes the model
+    assert np.abs(ref.les_fields()[0] - cso).max() > 1e-6 * np.abs(cso).max()
+    Qc, _ = o.state()
+    Qg, _ = g.state()
+    g.close()
+    assert state_err(Qg, Qc) <= STATE_TOL
 
 
 @pytest.mark.parametrize("bc,iso,les", [
