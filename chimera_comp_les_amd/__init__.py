@@ -27,6 +27,8 @@ LAYOUT_GLOBAL = 0
 LAYOUT_LOCAL = 1
 PREC_FP64 = 0
 PREC_FP32BULK = 1
+# bodies with more interpolation nodes are refused by cles_ibm_forces_noalloc
+FORCE_NOALLOC_MAX_NODES = 16384
 # return codes of include/cles.h
 OK, EINVAL, ENOMEM, EDEVICE, ESTATE = 0, -1, -2, -3, -4
 
@@ -65,7 +67,7 @@ EXPORTS = [
     "cles_batch_used", "cles_batch_set_tile", "cles_batch_advance", "cles_batch_time_step",
     "cles_batch_last_error", "cles_set_tile", "cles_batch_launch", "cles_batch_wait",
     "cles_set_state_qp", "cles_write_fields", "cles_read_fields", "cles_petsc_vec_write", "cles_petsc_vec_read",
-    "cles_get_averages", "cles_ibm_forces", "cles_write_vtk",
+    "cles_get_averages", "cles_ibm_forces", "cles_write_vtk", "cles_ibm_forces_noalloc",
 ]
 
 IBM_INFO_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
@@ -148,6 +150,7 @@ def load_library(path=LIB_PATH):
     L.cles_set_state_qp.argtypes = [vp, dp, dp, C.c_int]
     L.cles_get_averages.argtypes = [vp, dp, dp, dp, dp]
     L.cles_ibm_forces.argtypes = [vp, dp, dp]
+    L.cles_ibm_forces_noalloc.argtypes = [vp, dp, dp]
     L.cles_write_fields.argtypes = [vp, C.c_char_p, C.c_int]
     L.cles_write_vtk.argtypes = [vp, C.c_char_p, C.c_int]
     L.cles_read_fields.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
@@ -373,6 +376,13 @@ class Solver:
         c = np.ascontiguousarray(center, dtype=np.float64)
         out = np.empty(19)
         self._check(self.L.cles_ibm_forces(self.h, _ptr(c), _ptr(out)))
+        return out
+
+    def ibm_forces_noalloc(self, center=(0.0, 0.0, 0.0)):
+        """ibm_forces without device allocation (static per-GPU scratch)."""
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        out = np.empty(19)
+        self._check(self.L.cles_ibm_forces_noalloc(self.h, _ptr(c), _ptr(out)))
         return out
 
     def averages(self):

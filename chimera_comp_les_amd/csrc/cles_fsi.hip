@@ -11,6 +11,8 @@
 // accumulation in list order with the reference's update order, which keeps
 // the sums bit-identical.  The function-scope r_x, r_y, r_z of the reference
 // keep the last face's values across nodes; they start at 0 here.
+#include <mutex>
+
 #include "cles_internal.hpp"
 
 namespace cles {
@@ -279,28 +281,13 @@ __global__ void force_sum_kernel(const double *rec, int n_ib, double X_c, double
 
 using namespace cles;
 
-extern "C" int cles_ibm_forces(cles_ctx *c, const double center[3], double out[CLES_FORCE_N]) {
-  if (!c || !center || !out) return CLES_EINVAL;
-  if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
+namespace cles {
+namespace {
+
+// Gather the per-node records into rec (kRec per list node) and fold them
+// into dout (CLES_FORCE_N), then copy the outputs to the host.
+int forces_run(cles_ctx *c, const double center[3], double out[CLES_FORCE_N], double *rec, double *dout) {
   cles_ibm_dev *b = c->ibm;
-  if (!b) return set_error(c, CLES_ESTATE, "no immersed body (cles_ibm_search)");
-  for (int q = 0; q < CLES_FORCE_N; q++) out[q] = 0.;
-  if (b->n_ib == 0) return CLES_OK;
-  // work buffers of the body's list, made at the first call and released
-  // with the body (a new cles_ibm_search makes a new list)
-  if (!b->frec) {
-    if (hipMalloc((void **)&b->frec, (size_t)b->n_ib * kRec * sizeof(double)) != hipSuccess) {
-      b->frec = nullptr;
-      return set_error(c, CLES_ENOMEM, "force work buffers");
-    }
-  }
-  if (!b->fout) {
-    if (hipMalloc((void **)&b->fout, CLES_FORCE_N * sizeof(double)) != hipSuccess) {
-      b->fout = nullptr;
-      return set_error(c, CLES_ENOMEM, "force work buffers");
-    }
-  }
-  double *rec = b->frec, *dout = b->fout;
   ForceArgs a;
   a.g = c->g;
   a.n_ib = b->n_ib;
@@ -326,4 +313,83 @@ extern "C" int cles_ibm_forces(cles_ctx *c, const double center[3], double out[C
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_error(c, CLES_EDEVICE, "surface forces: %s", hipGetErrorString(e));
   return CLES_OK;
+}
+
+}  // namespace
+}  // namespace cles
+
+extern "C" int cles_ibm_forces(cles_ctx *c, const double center[3], double out[CLES_FORCE_N]) {
+  if (!c || !center || !out) return CLES_EINVAL;
+  if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
+  cles_ibm_dev *b = c->ibm;
+  if (!b) return set_error(c, CLES_ESTATE, "no immersed body (cles_ibm_search)");
+  for (int q = 0; q < CLES_FORCE_N; q++) out[q] = 0.;
+  if (b->n_ib == 0) return CLES_OK;
+  // work buffers of the body's list, made at the first call and released
+  // with the body (a new cles_ibm_search makes a new list)
+  if (!b->frec) {
+    if (hipMalloc((void **)&b->frec, (size_t)b->n_ib * kRec * sizeof(double)) != hipSuccess) {
+      b->frec = nullptr;
+      return set_error(c, CLES_ENOMEM, "force work buffers");
+    }
+  }
+  if (!b->fout) {
+    if (hipMalloc((void **)&b->fout, CLES_FORCE_N * sizeof(double)) != hipSuccess) {
+      b->fout = nullptr;
+      return set_error(c, CLES_ENOMEM, "force work buffers");
+    }
+  }
+  return forces_run(c, center, out, b->frec, b->fout);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form.  The records and outputs live in a file-scope device
+// array: every GPU the library runs on holds its own copy of it (about 7 MB)
+// until the process ends.  Its device address is looked up once per device
+// with that device current; host code passes that address, never the
+// array's name, to the kernels and copies.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchDoubles = (size_t)CLES_FORCE_NOALLOC_MAX_NODES * cles::kRec + CLES_FORCE_N;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static int scratch_address(cles_ctx *c, double **ptr) {
+  if (c->dev < 0 || c->dev >= kMaxDevices) return cles::set_error(c, CLES_EINVAL, "device %d out of range", c->dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[c->dev]) {
+    int cur = -1;
+    CLES_HIP(c, hipGetDevice(&cur));
+    if (cur != c->dev) CLES_HIP(c, hipSetDevice(c->dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != c->dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return cles::set_error(c, CLES_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[c->dev] = (double *)p;
+  }
+  *ptr = g_scratch_ptr[c->dev];
+  return CLES_OK;
+}
+
+extern "C" int cles_ibm_forces_noalloc(cles_ctx *c, const double center[3], double out[CLES_FORCE_N]) {
+  if (!c || !center || !out) return CLES_EINVAL;
+  if (!c->have_state) return cles::set_error(c, CLES_ESTATE, "no state uploaded");
+  cles_ibm_dev *b = c->ibm;
+  if (!b) return cles::set_error(c, CLES_ESTATE, "no immersed body (cles_ibm_search)");
+  for (int q = 0; q < CLES_FORCE_N; q++) out[q] = 0.;
+  if (b->n_ib == 0) return CLES_OK;
+  if (b->n_ib > CLES_FORCE_NOALLOC_MAX_NODES)
+    return cles::set_error(c, CLES_EINVAL,
+                           "cles_ibm_forces_noalloc: %d interpolation nodes, more than the %d its static "
+                           "scratch holds (use cles_ibm_forces)", b->n_ib, CLES_FORCE_NOALLOC_MAX_NODES);
+  double *base = nullptr;
+  int r = scratch_address(c, &base);
+  if (r) return r;
+  // every record field the fold reads is written by the gather of this call
+  return cles::forces_run(c, center, out, base, base + (size_t)CLES_FORCE_NOALLOC_MAX_NODES * cles::kRec);
 }

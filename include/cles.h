@@ -142,6 +142,17 @@ int cles_ibm_get_list(cles_ctx *c, cles_ibm_info *out, int n);
  * M x,y,z; Mdpdn x,y,z; Ia x,y,z. */
 #define CLES_FORCE_N 19
 int cles_ibm_forces(cles_ctx *c, const double center[3], double out[CLES_FORCE_N]);
+/* The same outputs, computed without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc), whereas
+ * cles_ibm_forces allocates its work buffers at its first call on a body.
+ * The per-node records live in a static device array that each GPU holds
+ * (about 7 MB) until the process ends, shared by every call on that GPU:
+ * calls that overlap on one GPU race.  Bodies with more than
+ * CLES_FORCE_NOALLOC_MAX_NODES interpolation nodes are rejected with
+ * CLES_EINVAL. */
+#define CLES_FORCE_NOALLOC_MAX_NODES 16384
+int cles_ibm_forces_noalloc(cles_ctx *c, const double center[3], double out[CLES_FORCE_N]);
 
 /* Upload the conserved state; sets Q, Qold and the pressure by EqOfState.
  * Replaces the tail of SetInitialCondition (rhs.c:424-433). */
