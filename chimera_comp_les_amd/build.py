@@ -22,11 +22,19 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # no fused multiply-add contraction: the FP64 path then reproduces the
-    # reference's operation order bit for bit (measured cost ~3% on the stage kernel)
-    "-ffp-contract=off",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ] + os.environ.get("CLES_EXTRA_FLAGS", "").split()
+
+# Fused multiply-add contraction per translation unit.  The right-hand side
+# and LES kernels contract a*b + c within each expression (the FP64 bar there
+# is 1e-12 relative, not bits); the immersed-body, surface-force, boundary,
+# statistics and I/O kernels keep the reference's operation order bit for bit
+# (bit-exact classification, stencils and forces).
+CONTRACT = {"cles_stage.hip": "on", "cles_les.hip": "on"}
+
+
+def contract_flag(src):
+    return "-ffp-contract=" + CONTRACT.get(os.path.basename(src), "off")
 
 
 def hipcc():
@@ -48,7 +56,7 @@ def build_lib(verbose=False, jobs=8):
     for src in sources():
         obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
         objs.append(obj)
-        cmd = [cc] + HIP_FLAGS + ["-c", src, "-o", obj]
+        cmd = [cc] + HIP_FLAGS + [contract_flag(src), "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

@@ -41,8 +41,27 @@ __host__ __device__ __forceinline__ long long nidw(const Grid &g, int k, int j, 
   return nid(g, wrapc(k, g.mz, g.pz), wrapc(j, g.my, g.py), wrapc(i, g.mx, g.px));
 }
 
+// 1/x without an IEEE division: the hardware reciprocal refined by two Newton
+// steps (float: one), a few instructions instead of the ~11 of a correctly
+// rounded division and within an ulp of the quotient.  x is a density, a
+// temperature or a sum of positive weights where it is used: finite, non-zero.
+__device__ __forceinline__ double rcp64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ float rcp32(float x) {
+  float y = __builtin_amdgcn_rcpf(x);
+  return fmaf(y, fmaf(-x, y, 1.0f), y);
+}
+__device__ __forceinline__ double rcpT(double x) { return rcp64(x); }
+__device__ __forceinline__ float rcpT(float x) { return rcp32(x); }
+
 // ---------------------------------------------------------------------------
-// Reconstruction primitives, in the operation order of rhs.c:2256-2568.
+// Reconstruction primitives, in the operation order of rhs.c:2256-2568
+// (WENO weights normalised with one reciprocal instead of four divisions).
 // T is the flux arithmetic type (double for the FP64 drop-in, float for the
 // FP32-bulk form).
 // ---------------------------------------------------------------------------
@@ -61,29 +80,24 @@ template <typename T> __device__ __forceinline__ T muscl_qr(T qm, T q0, T qp) {
   T S = minmod(qp - q0, q0 - qm);
   return q0 - T(0.5) * S;
 }
-template <typename T> __device__ __forceinline__ T weno_ql(T qm, T q0, T qp) {
+// WENO3 with the truncated linear weights d0 = 0.6666667, d1 = 0.3333333 and
+// eps = 1e-6 (rhs.c:2359-2461): w0 = (d0 / b0^2) / (d0 / b0^2 + d1 / b1^2)
+// with b = eps + beta, formed as A / (A + B), A = d0 b1^2, B = d1 b0^2.
+template <typename T> __device__ __forceinline__ T weno_mix(T Qr0, T Qr1, T beta0, T beta1) {
   const T d0 = T(0.6666667), d1 = T(0.3333333), eps = T(1e-6);
+  const T e0 = eps + beta0, e1 = eps + beta1;
+  const T A = d0 * (e1 * e1), B = d1 * (e0 * e0);
+  return (A * Qr0 + B * Qr1) * rcpT(A + B);
+}
+template <typename T> __device__ __forceinline__ T weno_ql(T qm, T q0, T qp) {
   T Qr0 = T(0.5) * q0 + T(0.5) * qp;
   T Qr1 = T(-0.5) * qm + T(1.5) * q0;
-  T beta0 = (qp - q0) * (qp - q0);
-  T beta1 = (qm - q0) * (qm - q0);
-  T alfa0 = d0 / ((eps + beta0) * (eps + beta0));
-  T alfa1 = d1 / ((eps + beta1) * (eps + beta1));
-  T alfasum = alfa0 + alfa1;
-  T w0 = alfa0 / alfasum, w1 = alfa1 / alfasum;
-  return w0 * Qr0 + w1 * Qr1;
+  return weno_mix(Qr0, Qr1, (qp - q0) * (qp - q0), (qm - q0) * (qm - q0));
 }
 template <typename T> __device__ __forceinline__ T weno_qr(T qm, T q0, T qp) {
-  const T d0 = T(0.6666667), d1 = T(0.3333333), eps = T(1e-6);
   T Qr0 = T(1.5) * q0 - T(0.5) * qp;
   T Qr1 = T(0.5) * qm + T(0.5) * q0;
-  T beta0 = (qp - q0) * (qp - q0);
-  T beta1 = (qm - q0) * (qm - q0);
-  T alfa0 = d0 / ((eps + beta0) * (eps + beta0));
-  T alfa1 = d1 / ((eps + beta1) * (eps + beta1));
-  T alfasum = alfa0 + alfa1;
-  T w0 = alfa0 / alfasum, w1 = alfa1 / alfasum;
-  return w0 * Qr0 + w1 * Qr1;
+  return weno_mix(Qr0, Qr1, (qp - q0) * (qp - q0), (qm - q0) * (qm - q0));
 }
 template <typename T> __device__ __forceinline__ T eno_qr(T qm, T q0, T qp, int k) {
   if (k == 2) return T(0.5) * q0 + T(0.5) * qp;
@@ -95,7 +109,7 @@ template <typename T> __device__ __forceinline__ T eno_ql(T qm, T q0, T qp, int 
 }
 // skew-symmetric 4-point product, rhs.c:2468-2474
 template <typename T> __device__ __forceinline__ T fp_2(const T u[4], const T v[4]) {
-  return (u[1] + u[2]) * (v[1] + v[2]) / T(3.0) -
+  return (u[1] + u[2]) * (v[1] + v[2]) * T(1.0 / 3.0) -
          T(1. / 24.) * (u[0] * v[0] + u[0] * v[2] + u[1] * v[1] + u[1] * v[3] + u[2] * v[2] + u[2] * v[0] +
                         u[3] * v[1] + u[3] * v[3]);
 }
@@ -106,26 +120,13 @@ template <typename T> __device__ __forceinline__ T fp4(const T u[4]) {
   return T(1.0 / 12.0) * (-u[0] + T(7) * u[1] + T(7) * u[2] - u[3]);
 }
 
-// pow(x, 1.5) of Sutherland's law (rhs.c:1636) rounded once: x*s plus the
-// exact low part of x*s and x*(sqrt(x) - s), s = RN(sqrt(x)); fma gives
-// x - s*s and the low half of x*s exactly, so the sum carries ~104 bits.
-__device__ __forceinline__ double pow15_cr(double x) {
-  if (x == 0.0) return 0.0;
-  const double s = sqrt(x);
-  const double r = fma(-s, s, x);
-  const double hi = x * s;
-  const double lo = fma(x, s, -hi);
-  const double corr = x * (r / (2.0 * s));
-  return hi + (lo + corr);
-}
-
 // Rusanov flux, gamma hard-coded to 1.4 as in rhs.c:2596 / 3256.
 template <typename T>
 __device__ __forceinline__ void rusanov(const T *L, const T *R, const T *h, T uconL, T uconR, T *F) {
   const T gamma = T(1.4);
   T gii = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
-  T CsoundL = gamma * L[5] / L[0];
-  T CsoundR = gamma * R[5] / R[0];
+  T CsoundL = gamma * L[5] * rcpT(L[0]);
+  T CsoundR = gamma * R[5] * rcpT(R[0]);
   T saL = fabs(uconL) + sqrt(fabs(CsoundL) * gii);
   T saR = fabs(uconR) + sqrt(fabs(CsoundR) * gii);
   T SA = pmax(saL, saR);
@@ -136,7 +137,8 @@ __device__ __forceinline__ void rusanov(const T *L, const T *R, const T *h, T uc
   F[4] = T(0.5) * ((L[5] + L[4]) * uconL + (R[5] + R[4]) * uconR) - T(0.5) * SA * (R[4] - L[4]);
 }
 template <typename T> __device__ __forceinline__ T ucon_of(const T *s, const T *h) {
-  return h[0] * (s[1] / s[0]) + h[1] * (s[2] / s[0]) + h[2] * (s[3] / s[0]);
+  const T inv = rcpT(s[0]);
+  return h[0] * (s[1] * inv) + h[1] * (s[2] * inv) + h[2] * (s[3] * inv);
 }
 
 // Analytic hybrid-flux weight when -calc_sigma is 0 (rhs.c:3328-3444):

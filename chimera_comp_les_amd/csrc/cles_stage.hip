@@ -34,46 +34,14 @@ namespace {
 
 constexpr int NT = 256;    // threads per workgroup of the larger tile shapes
 // occupancy target of the stage kernels (waves per SIMD the register
-// allocation must allow)
-#ifndef CLES_STAGE_WAVES
-#define CLES_STAGE_WAVES 2
-#endif
-// convective and viscous flux of a face computed in one pass (one barrier
-// pair per direction instead of two): 0 never, 1 always, 2 all forms but the
-// LES forms (measured faster there as two passes)
-#ifndef CLES_STAGE_MERGE
-#define CLES_STAGE_MERGE 2
-#endif
-// issue the viscous face loads of a direction early (forms with two passes
-// per direction): 0 no, 1 at the top of the direction, 2 after its
-// convective faces; PF_PART selects the loads: 1 metrics and aj, 2 the
-// others (rho, p, nu_t, filter width), 3 all (1 and 3 spill registers)
-#ifndef CLES_STAGE_PREFETCH
-#define CLES_STAGE_PREFETCH 2
-#endif
-#ifndef CLES_STAGE_PF_PART
-#define CLES_STAGE_PF_PART 2
-#endif
-// staging loops issue the loads of all their iterations before the LDS
-// stores: bit 0 the direction's node data, bit 1 the primitive window
-#ifndef CLES_STAGE_BATCH
-#define CLES_STAGE_BATCH 2
-#endif
-// load the node's update inputs (aj, state, old state) while the last
-// direction's viscous faces are evaluated
-#ifndef CLES_STAGE_PF_END
-#define CLES_STAGE_PF_END 0
-#endif
-// LES forms: load the face densities of the LES energy term at the top of
-// each direction
-#ifndef CLES_STAGE_PF_LESE
-#define CLES_STAGE_PF_LESE 1
-#endif
-// order of the tiles along an XCD's range: 1 = x, z, y; 0 = x, y, z
-#ifndef CLES_TILE_XZY
-#define CLES_TILE_XZY 1
-#endif
+// allocation must allow); 3 spills and runs slower
+constexpr int kStageWaves = 2;
 constexpr int NSLOT = 64;  // spread of the inf-norm atomics
+
+// contravariant velocity (m . rhoU) / rho of the four-point central flux
+__device__ __forceinline__ double contra(double m0, double m1, double m2, double ru, double rv, double rw, double rd) {
+  return (m0 * ru + m1 * rv + m2 * rw) * rcp64(rd);
+}
 
 // Tile of interior nodes owned by one workgroup, with the face counts of its
 // convective/viscous face buffer and its primitive window (one-node halo).
@@ -113,6 +81,9 @@ struct StageArgs {
   // uniform parts of the viscosity, formed once on the host with the same
   // operations: 1/ren, 110.4/Tref, (1/ren) (1 + 110.4/Tref)
   double inv_ren, suth_b, suth_a;
+  // gamma Ma^2 (Sutherland's temperature) and the heat-flux factor
+  // gamma / (Pr (gamma - 1)), formed once on the host
+  double gm2, kheat;
   // reference state of the FP32-bulk form: the float arithmetic carries the
   // deviations from it, the reference parts of the fluxes stay in double
   double rho_ref, p_ref, e_ref, h_ref, t_ref;
@@ -217,14 +188,8 @@ __device__ __forceinline__ void face_metric(const StageArgs &a, int d, long long
   h[2] = T(0.5 * (m[n0 + 2 * np] + m[n1 + 2 * np]));
 }
 
-// 1/rho to ~1e-15 relative without a double division: float reciprocal and
-// one Newton step in double
-__device__ __forceinline__ double inv_refined(double r) {
-  double y = (double)(1.0f / (float)r);
-  return fma(y, fma(-r, y, 1.0), y);
-}
 __device__ __forceinline__ double ucon_d(const double *s, const double *h) {
-  const double inv = inv_refined(s[0]);
+  const double inv = rcp64(s[0]);
   return h[0] * (s[1] * inv) + h[1] * (s[2] * inv) + h[2] * (s[3] * inv);
 }
 
@@ -237,8 +202,8 @@ __device__ void rusanov_split(const StageArgs &a, const double *L, const double 
                               double uconR, double F[5]) {
   const float h0 = (float)hd[0], h1 = (float)hd[1], h2 = (float)hd[2];
   const float gii = h0 * h0 + h1 * h1 + h2 * h2;
-  const float CsoundL = 1.4f * (float)L[5] / (float)L[0];
-  const float CsoundR = 1.4f * (float)R[5] / (float)R[0];
+  const float CsoundL = 1.4f * (float)L[5] * rcp32((float)L[0]);
+  const float CsoundR = 1.4f * (float)R[5] * rcp32((float)R[0]);
   const float uL = (float)uconL, uR = (float)uconR;
   const float saL = fabsf(uL) + sqrtf(fabsf(CsoundL) * gii);
   const float saR = fabsf(uR) + sqrtf(fabsf(CsoundR) * gii);
@@ -385,7 +350,7 @@ __device__ bool conv_face_les_split(const StageArgs &a, const Sv &sv, int d, int
     } else {
       rd = a.q[n]; ru = a.q[n + np]; rv = a.q[n + 2 * np]; rw = a.q[n + 3 * np];
       ed = a.q[n + 4 * np]; pd = a.p[n];
-      Ud[q] = (mt[n] * ru + mt[n + np] * rv + mt[n + 2 * np] * rw) * inv_refined(rd);
+      Ud[q] = contra(mt[n], mt[n + np], mt[n + 2 * np], ru, rv, rw, rd);
     }
     U[q] = (float)Ud[q];
     r[q] = (float)(rd - a.rho_ref);
@@ -469,7 +434,7 @@ __device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, in
           w[q] = T(a.q[n + 3 * np]);
           e = T(a.q[n + 4 * np]);
           pp[q] = T(a.p[n]);
-          U[q] = T(1.) / r[q] * (T(mt[n]) * u[q] + T(mt[n + np]) * v[q] + T(mt[n + 2 * np]) * w[q]);
+          U[q] = T(contra(mt[n], mt[n + np], mt[n + 2 * np], double(u[q]), double(v[q]), double(w[q]), double(r[q])));
         }
         ep[q] = d == 2 ? e + pp[q] : pp[q] + e;
       }
@@ -497,7 +462,7 @@ __device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, in
     T vc = T(0.5) * (s0[2] + s1[2]);
     T wc = T(0.5) * (s0[3] + s1[3]);
     T ec = T(0.5) * (s0[4] + s1[4]);
-    T Uc = (h[0] * uc + h[1] * vc + h[2] * wc) / rc;
+    T Uc = (h[0] * uc + h[1] * vc + h[2] * wc) * T(rcp64(rc));
     T pc = T(0.5) * (s0[5] + s1[5]);
     Fc[0] = rc * Uc;
     Fc[1] = uc * Uc + pc * h[0];
@@ -681,25 +646,20 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
   T nu = T(a.inv_ren);
   if (LES) nu = T(a.inv_ren * 0.5 * (rA + v.rA1));  // rho at [i] and [i+1] on every face
   if (a.suth) {
-    double Tr = a.gamma * a.Ma * a.Ma * (v.p[0] + v.p[1]) / (rA + rB);
-    if (Tr < 0)
-      nu = T(a.inv_ren);
-    else
-      nu = T(a.suth_a / (Tr + a.suth_b) * pow15_cr(Tr));  // 1/ren (1 + C/Tref) / (Tr + C/Tref) Tr^1.5
+    // Sutherland (rhs.c:1628-1635): Tr = gamma Ma^2 p_face / rho_face,
+    // nu = 1/ren (1 + C/Tref) / (Tr + C/Tref) Tr^1.5
+    const double Tr = a.gm2 * (v.p[0] + v.p[1]) * rcp64(rA + rB);
+    nu = Tr < 0 ? T(a.inv_ren) : T(a.suth_a * rcp64(Tr + a.suth_b) * (Tr * sqrt(Tr)));
   }
   T div = T(2. / 3.) * (dudc * csi0 + dude * eta0 + dudz * zet0 + dvdc * csi1 + dvde * eta1 + dvdz * zet1 +
                         dwdc * csi2 + dwde * eta2 + dwdz * zet2);
-  T fu = (g11 * dudc + g21 * dude + g31 * dudz + r11 * m0 + r21 * m1 + r31 * m2) * ajc * (nu);
-  T fv = (g11 * dvdc + g21 * dvde + g31 * dvdz + r12 * m0 + r22 * m1 + r32 * m2) * ajc * (nu);
-  T fw = (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * m0 + r23 * m1 + r33 * m2) * ajc * (nu);
-  if (d == 2) {
-    fu = T(0.0) + fu;
-    fv = T(0.0) + fv;
-    fw = T(0.0) + fw;
-  }
-  fu -= div * m0 * ajc * nu;
-  fv -= div * m1 * ajc * nu;
-  fw -= div * m2 * ajc * nu;
+  // deviatoric stress contractions, shared by the molecular and the SGS
+  // stress: (g . grad u + r . m - div m) of rhs.c:1646-1655, 1674-1679
+  const T bu = g11 * dudc + g21 * dude + g31 * dudz + r11 * m0 + r21 * m1 + r31 * m2 - div * m0;
+  const T bv = g11 * dvdc + g21 * dvde + g31 * dvdz + r12 * m0 + r22 * m1 + r32 * m2 - div * m1;
+  const T bw = g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * m0 + r23 * m1 + r33 * m2 - div * m2;
+  const T cm = ajc * nu;
+  T fu = bu * cm, fv = bv * cm, fw = bw * cm;
   const int id = d == 0 ? i : d == 1 ? j : k;
   const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
   T uh0, uh1, uh2;
@@ -712,25 +672,24 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
     uh1 = T(0.5) * (V[wB] + V[wA]);
     uh2 = T(0.5) * (Wz[wB] + Wz[wA]);
   }
-  const T gam = T(a.gamma), pr = T(a.Pr);
-  T fe = (fu * uh0 + fv * uh1 + fw * uh2) + (g11 * dT[0] + g21 * dT[1] + g31 * dT[2]) * ajc * (gam * nu / pr / (gam - T(1.0)));
+  // heat flux with the conductivity gamma nu / (Pr (gamma - 1)) (rhs.c:1666-1670)
+  T fe = (fu * uh0 + fv * uh1 + fw * uh2) + (g11 * dT[0] + g21 * dT[1] + g31 * dT[2]) * (cm * T(a.kheat));
   if (LES) {
-    const T CI = T(0.06);
+    constexpr double kIso = 0.06 * 2.0 / 3.0;  // 2/3 CI, CI = 0.06 (rhs.c:13)
     T sA = T(sigma_at(a, d, k, j, i));
     T sB = T(sigma_at(a, d, k + (d == 2), j + (d == 1), i + (d == 0)));
     T nu_t = T(0.5) * (T(v.nut[0]) * sA + T(v.nut[1]) * sB);
     T rho_c = T(0.5 * (rA + rB));
     T sigma_c = T(0.5) * (sA + sB);
     if ((a.bc[2 * d] == 1 && id == 0) || (a.bc[2 * d + 1] == 1 && id == m - 2)) nu_t = T(0);
-    T tx = (g11 * dudc + g21 * dude + g31 * dudz + r11 * m0 + r21 * m1 + r31 * m2 - div * m0) * ajc * (nu_t * rho_c);
-    T ty = (g11 * dvdc + g21 * dvde + g31 * dvdz + r12 * m0 + r22 * m1 + r32 * m2 - div * m1) * ajc * (nu_t * rho_c);
-    T tz = (g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * m0 + r23 * m1 + r33 * m2 - div * m2) * ajc * (nu_t * rho_c);
+    const T ct = ajc * (nu_t * rho_c);
+    T tx = bu * ct, ty = bv * ct, tz = bw * ct;
     T S2 = T(.5) * ((r11 - div) * (r11 - div) + r12 * r12 + r13 * r13 + r21 * r21 + (r22 - div) * (r22 - div) +
                     r23 * r23 + r31 * r31 + r32 * r32 + (r33 - div) * (r33 - div));
     T f2 = T(v.f2);  // (pow(1/aj, 1/3))^2 at the face's lower node
-    T Ssx = -S2 * f2 * CI * T(2.0) / T(3.0) * m0 * rho_c * sigma_c;
-    T Ssy = -S2 * f2 * CI * T(2.0) / T(3.0) * m1 * rho_c * sigma_c;
-    T Ssz = -S2 * f2 * CI * T(2.0) / T(3.0) * m2 * rho_c * sigma_c;
+    // isotropic part -2/3 CI rho Delta^2 S2 sigma, without the face Jacobian (rhs.c:1693-1697)
+    const T ss = S2 * f2 * T(kIso) * rho_c * sigma_c;
+    T Ssx = -ss * m0, Ssy = -ss * m1, Ssz = -ss * m2;
     fu += tx + Ssx;
     fv += ty + Ssy;
     fw += tz + Ssz;
@@ -780,9 +739,12 @@ template <class TL> __device__ __forceinline__ int fk_of(int lx, int ly, int lz)
 template <class TL, typename T, bool LES, bool VISC, int MODE>
 __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) {
   constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, WX = TL::WX, WY = TL::WY, NW = TL::NW;
+  // The convective and viscous flux of a face are computed in one pass (one
+  // barrier pair per direction) except in the LES forms, where two passes
+  // through a smaller face buffer measured faster.
+  constexpr bool MERGE = VISC && !LES;
   // face buffer: the convective flux (5) and, computed in the same pass, the
   // viscous flux (4) and the LES stresses (3) of each face of one direction
-  constexpr bool MERGE = VISC && (CLES_STAGE_MERGE == 1 || (CLES_STAGE_MERGE == 2 && !LES));
   constexpr int FV = VISC ? (LES ? 7 : 4) : 0;
   constexpr int FC = MERGE ? 5 + FV : (VISC ? (LES ? 7 : 5) : 5);
   constexpr int VO = MERGE ? 5 : 0;  // first viscous slot
@@ -802,13 +764,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   __shared__ double Ss[STG ? NCH * NSM : 1];
   const Grid &g = a.g;
   if (tile >= a.ntx * a.nty * a.ntz) return;
-#if CLES_TILE_XZY
   // x fastest, then z, then y: the tiles in flight on an XCD share their z
   // halos (the deepest: a 3-node tile with +-1 and +-2 stencils) in its L2
   const int tix = tile % a.ntx, tiz = (tile / a.ntx) % a.ntz, tiy = tile / (a.ntx * a.ntz);
-#else
-  const int tix = tile % a.ntx, tiy = (tile / a.ntx) % a.nty, tiz = tile / (a.ntx * a.nty);
-#endif
   const int i0 = 1 + tix * TX, j0 = 1 + tiy * TY, k0 = 1 + tiz * TZ;
   const int t = threadIdx.x;
   const long long np = g.np;
@@ -829,58 +787,33 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   const bool les_e = LES && VISC && valid && gj > 2 && gj < g.my - 2 && !(a.has_ib && !(a.nv[n] < 0.5));
   T E0 = T(0), E1 = T(0), E2 = T(0);
   Win win{i0 - 1, j0 - 1, k0 - 1};
-  // the node's update inputs, loaded early (CLES_STAGE_PF_END) or at the end
-  double ajn = 0., qin[5] = {0., 0., 0., 0., 0.}, qol[5] = {0., 0., 0., 0., 0.};
-  auto load_end = [&]() {
-    if (!valid) return;
-    ajn = a.aj[n];
-#pragma unroll
-    for (int q = 0; q < 5; q++) qin[q] = a.q[n + q * np];
-    if (MODE != 0) {
-#pragma unroll
-      for (int q = 0; q < 5; q++) qol[q] = a.qold[n + q * np];
-    }
-  };
   if (VISC) {
-    // stage u, v, w, p/rho of the tile and a one-node halo
-    if (CLES_STAGE_BATCH & 2) {
-      constexpr int NIT = (NW + TL::NT - 1) / TL::NT;
-      double qv[NIT][5];
-      bool ok[NIT];
+    // stage u, v, w, p/rho of the tile and a one-node halo; the loads of all
+    // slots are issued before any LDS store, so they are in flight together
+    constexpr int NIT = (NW + TL::NT - 1) / TL::NT;
+    double qv[NIT][5];
+    bool ok[NIT];
 #pragma unroll
-      for (int it = 0; it < NIT; it++) {
-        const int w = t + it * TL::NT;
-        const int ii = win.i0 + w % WX, jj = win.j0 + (w / WX) % WY, kk = win.k0 + w / (WX * WY);
-        ok[it] = w < NW && ii <= g.mx - 1 && jj <= g.my - 1 && kk <= g.mz - 1;
-        if (ok[it]) {
-          const long long m = nid(g, kk, jj, ii);
+    for (int it = 0; it < NIT; it++) {
+      const int w = t + it * TL::NT;
+      const int ii = win.i0 + w % WX, jj = win.j0 + (w / WX) % WY, kk = win.k0 + w / (WX * WY);
+      ok[it] = w < NW && ii <= g.mx - 1 && jj <= g.my - 1 && kk <= g.mz - 1;
+      if (ok[it]) {
+        const long long m = nid(g, kk, jj, ii);
 #pragma unroll
-          for (int c = 0; c < 4; c++) qv[it][c] = a.q[m + c * np];
-          qv[it][4] = a.p[m];
-        }
+        for (int c = 0; c < 4; c++) qv[it][c] = a.q[m + c * np];
+        qv[it][4] = a.p[m];
       }
+    }
 #pragma unroll
-      for (int it = 0; it < NIT; it++) {
-        const int w = t + it * TL::NT;
-        if (!ok[it]) continue;
-        const double r = qv[it][0];
-        Ws[w] = a.twoD == 1 ? T(0) : T(qv[it][1] / r);  // CalcUCat, rhs.c:1349
-        Ws[NW + w] = T(qv[it][2] / r);
-        Ws[2 * NW + w] = T(qv[it][3] / r);
-        Ws[3 * NW + w] = T(qv[it][4] / r - a.t_ref);
-      }
-    } else {
-      for (int w = t; w < NW; w += TL::NT) {
-        int wx = w % WX, wy = (w / WX) % WY, wz = w / (WX * WY);
-        int ii = win.i0 + wx, jj = win.j0 + wy, kk = win.k0 + wz;
-        if (ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
-        long long m = nid(g, kk, jj, ii);
-        double r = a.q[m];
-        Ws[w] = a.twoD == 1 ? T(0) : T(a.q[m + np] / r);
-        Ws[NW + w] = T(a.q[m + 2 * np] / r);
-        Ws[2 * NW + w] = T(a.q[m + 3 * np] / r);
-        Ws[3 * NW + w] = T(a.p[m] / r - a.t_ref);  // t_ref = 0 in FP64: only gradients of p/rho are used
-      }
+    for (int it = 0; it < NIT; it++) {
+      const int w = t + it * TL::NT;
+      if (!ok[it]) continue;
+      const double ir = rcp64(qv[it][0]);
+      Ws[w] = a.twoD == 1 ? T(0) : T(qv[it][1] * ir);  // CalcUCat, rhs.c:1349
+      Ws[NW + w] = T(qv[it][2] * ir);
+      Ws[2 * NW + w] = T(qv[it][3] * ir);
+      Ws[3 * NW + w] = T(qv[it][4] * ir - a.t_ref);  // t_ref = 0 in FP64: only gradients of p/rho are used
     }
     __syncthreads();  // the window is read by other threads' viscous faces
   }
@@ -892,23 +825,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     const int lo = (lz * ey + ly) * ex + lx;
     const int hi = lo + (d == 0 ? 1 : d == 1 ? ex : ex * ey);
     // Two-pass forms whose faces of every direction fit one pass of the
-    // workgroup: the viscous face of this thread (f = t) loads its inputs
-    // before the convective sums, and the loads stay in flight through them
-    // (barriers wait for LDS traffic only).
-    constexpr bool PF = VISC && !MERGE && CLES_STAGE_PREFETCH && TL::NFI <= TL::NT && TL::NFJ <= TL::NT &&
-                        TL::NFK <= TL::NT;
+    // workgroup: the non-metric inputs of this thread's viscous face (f = t)
+    // are loaded right after its convective face, and stay in flight through
+    // the convective sums (barriers wait for LDS traffic only); loading the
+    // metric part there too spills registers.
+    constexpr bool PF = VISC && !MERGE && TL::NFI <= TL::NT && TL::NFJ <= TL::NT && TL::NFK <= TL::NT;
     VisIn vpre;
-    auto prefetch = [&]() {
-      if (PF && t < nfd) {
-        const int fx = t % ex, fy = (t / ex) % ey, fz = t / (ex * ey);
-        const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
-        if (fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2) visc_load<LES, CLES_STAGE_PF_PART>(a, d, fk, fj, fi, vpre);
-      }
-    };
-    if (CLES_STAGE_PREFETCH == 1) prefetch();
     // face densities of the LES energy term: rho at n, n - st, n + st along d
     double rle[3];
-    if (CLES_STAGE_PF_LESE && les_e) {
+    if (les_e) {
       const long long st = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
       rle[0] = a.q[n];
       rle[1] = a.q[n - st];
@@ -920,45 +845,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     const int bx = TX + 4 * (d == 0), by = TY + 4 * (d == 1), bz = TZ + 4 * (d == 2);
     const int ox = i0 - 2 * (d == 0), oy = j0 - 2 * (d == 1), oz = k0 - 2 * (d == 2);
     constexpr int ns = NSM;
-    if (STG && (CLES_STAGE_BATCH & 1)) {
-      const int nb = bx * by * bz;
-      const double *mt = metric(a, d);
-      constexpr int NIT = (NSM + TL::NT - 1) / TL::NT;
-      double sv9[NIT][9];
-      bool ok[NIT];
-#pragma unroll
-      for (int it = 0; it < NIT; it++) {
-        const int w = t + it * TL::NT;
-        const int ii = ox + w % bx, jj = oy + (w / bx) % by, kk = oz + w / (bx * by);
-        ok[it] = w < nb && ii >= 0 && jj >= 0 && kk >= 0 && ii <= g.mx - 1 && jj <= g.my - 1 && kk <= g.mz - 1;
-        if (ok[it]) {
-          const long long m = nid(g, kk, jj, ii);
-#pragma unroll
-          for (int c = 0; c < 5; c++) sv9[it][c] = a.q[m + c * np];
-          sv9[it][5] = a.p[m];
-          if (LES) {
-            sv9[it][6] = mt[m];
-            sv9[it][7] = mt[m + np];
-            sv9[it][8] = mt[m + 2 * np];
-          }
-        }
-      }
-#pragma unroll
-      for (int it = 0; it < NIT; it++) {
-        const int w = t + it * TL::NT;
-        if (!ok[it]) continue;
-        const double rd = sv9[it][0], ru = sv9[it][1], rv = sv9[it][2], rw = sv9[it][3];
-#pragma unroll
-        for (int c = 0; c < 6; c++) Ss[c * ns + w] = sv9[it][c];
-        if (LES) {
-          if constexpr (sizeof(T) == 4)
-            Ss[6 * ns + w] = (sv9[it][6] * ru + sv9[it][7] * rv + sv9[it][8] * rw) * inv_refined(rd);
-          else
-            Ss[6 * ns + w] = 1. / rd * (sv9[it][6] * ru + sv9[it][7] * rv + sv9[it][8] * rw);
-        }
-      }
-      __syncthreads();
-    } else if (STG) {
+    if (STG) {
       const int nb = bx * by * bz;
       const double *mt = metric(a, d);
       for (int w = t; w < nb; w += TL::NT) {
@@ -972,13 +859,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         Ss[3 * ns + w] = rw;
         Ss[4 * ns + w] = a.q[m + 4 * np];
         Ss[5 * ns + w] = a.p[m];
-        if (LES) {
-          // the expressions of the four-point flux (double, or the FP32-bulk split form)
-          if constexpr (sizeof(T) == 4)
-            Ss[6 * ns + w] = (mt[m] * ru + mt[m + np] * rv + mt[m + 2 * np] * rw) * inv_refined(rd);
-          else
-            Ss[6 * ns + w] = 1. / rd * (mt[m] * ru + mt[m + np] * rv + mt[m + 2 * np] * rw);
-        }
+        // the contravariant velocity of the four-point flux (conv_face_les)
+        Ss[6 * ns + w] = contra(mt[m], mt[m + np], mt[m + 2 * np], ru, rv, rw, rd);
       }
       __syncthreads();
     }
@@ -1021,8 +903,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         }
       }
     }
-    if (CLES_STAGE_PREFETCH == 2) prefetch();
-    if (CLES_STAGE_PF_END && d == 2) load_end();
+    if (PF && t < nfd) {
+      const int fx = t % ex, fy = (t / ex) % ey, fz = t / (ex * ey);
+      const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
+      if (fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2) visc_load<LES, 2>(a, d, fk, fj, fi, vpre);
+    }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 5; q++) conv[q] = d == 0 ? Fs[q][hi] - Fs[q][lo] : conv[q] + Fs[q][hi] - Fs[q][lo];
@@ -1044,7 +929,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         }
         T G[4], al[3] = {T(0), T(0), T(0)};
         if (PF) {
-          if (CLES_STAGE_PF_PART != 3) visc_load<LES, 3 - CLES_STAGE_PF_PART>(a, d, fk, fj, fi, vpre);
+          visc_load<LES, 1>(a, d, fk, fj, fi, vpre);
           visc_face<TL, T, LES>(a, Ws, win, d, fk, fj, fi, vpre, G, al);
         } else {
           VisIn vi;
@@ -1065,18 +950,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       visc[q + 1] = d == 0 ? Fs[VO + q][hi] - Fs[VO + q][lo] : visc[q + 1] + Fs[VO + q][hi] - Fs[VO + q][lo];
     if (les_e) {
       // face densities (r_n + r_n-1)/2, (r_n+1 + r_n)/2 along d
-      double r0, rm, rp;
-      if (CLES_STAGE_PF_LESE) {
-        r0 = rle[0];
-        rm = rle[1];
-        rp = rle[2];
-      } else {
-        const long long st = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
-        r0 = a.q[n];
-        rm = a.q[n - st];
-        rp = a.q[n + st];
-      }
-      const T rl = T((r0 + rm) * 0.5), rh = T((rp + r0) * 0.5);
+      const T rl = T((rle[0] + rle[1]) * 0.5), rh = T((rle[2] + rle[0]) * 0.5);
       const T p0 = rh * T(Fs[VO + 4][hi]), p1 = rh * T(Fs[VO + 5][hi]), p2 = rh * T(Fs[VO + 6][hi]);
       E0 = d == 0 ? p0 - rl * T(Fs[VO + 4][lo]) : E0 + p0 - rl * T(Fs[VO + 4][lo]);
       E1 = d == 0 ? p1 - rl * T(Fs[VO + 5][lo]) : E1 + p1 - rl * T(Fs[VO + 5][lo]);
@@ -1088,18 +962,28 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     conv[1] = 0.;
     visc[1] = 0.;
   }
+  // the node's update inputs
+  double ajn = 0., qin[5] = {0., 0., 0., 0., 0.}, qol[5] = {0., 0., 0., 0., 0.};
+  if (valid) {
+    ajn = a.aj[n];
+#pragma unroll
+    for (int q = 0; q < 5; q++) qin[q] = a.q[n + q * np];
+    if (MODE != 0) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) qol[q] = a.qold[n + q * np];
+    }
+  }
   if (LES && a.ratio != 0.0 && valid) {
-    conv[3] = conv[3] - a.ratio / a.aj[n] * a.q[n] / a.dt;
-    conv[4] = conv[4] - a.ratio / a.aj[n] * a.q[n + 3 * np] / a.dt;
+    conv[3] = conv[3] - a.ratio / ajn * qin[0] / a.dt;
+    conv[4] = conv[4] - a.ratio / ajn * qin[3] / a.dt;
   }
   if (les_e) {
-    const double rn = a.q[n];
-    const T u0 = T(a.q[n + np] / rn), u1 = T(a.q[n + 2 * np] / rn), u2 = T(a.q[n + 3 * np] / rn);
+    const double ir = rcp64(qin[0]);
+    const T u0 = T(qin[1] * ir), u1 = T(qin[2] * ir), u2 = T(qin[3] * ir);
     const T e = u0 * E0 + u1 * E1 + u2 * E2;
     visc[4] = visc[4] + double(e);
   }
-  // ---- phase D2: Rhs = aj (Visc - Conv) (MultAj, rhs.c:4611-4629)
-  if (!CLES_STAGE_PF_END) load_end();
+  // ---- Rhs = aj (Visc - Conv) (MultAj, rhs.c:4611-4629)
   double rhs[5] = {0., 0., 0., 0., 0.};
   if (valid) {
     const bool fluid = !a.has_ib || a.nv[n] < 0.1;
@@ -1143,11 +1027,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
 #pragma unroll
   for (int q = 0; q < 5; q++) a.qo[n + q * np] = qn[q];
   // EqOfState (rhs.c:4853-4857)
-  a.po[n] = (a.gamma - 1) * (qn[4] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) / qn[0]);
+  a.po[n] = (a.gamma - 1) * (qn[4] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) * rcp64(qn[0]));
 }
 
 template <class TL, typename T, bool LES, bool VISC, int MODE>
-__global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(CLES_STAGE_WAVES))) void stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWaves))) void stage_kernel(StageArgs a) {
   // XCD-aware tile numbering: blockIdx % 8 selects the XCD, each XCD walks a
   // contiguous range of tiles and keeps their shared halos in its own L2.
   const int b = blockIdx.x;
@@ -1158,7 +1042,7 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(CLES_STA
 // independent grids; tstart[r] is the first workgroup of grid r (prefix sum
 // of the tile counts, tstart[nreq] = total).
 template <class TL, typename T, bool LES, bool VISC, int MODE>
-__global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(CLES_STAGE_WAVES))) void stage_batch_kernel(const StageArgs *__restrict__ tab, const int *__restrict__ tstart,
+__global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWaves))) void stage_batch_kernel(const StageArgs *__restrict__ tab, const int *__restrict__ tstart,
                                                          int nreq) {
   const int b = blockIdx.x;
   int lo = 0, hi = nreq;  // largest r with tstart[r] <= b
@@ -1238,6 +1122,8 @@ void fill_stage_args(cles_ctx *c, int in, int out, int istage, const double coef
     a.suth_b = C_sutherland / p.Tref;
     a.suth_a = 1. / p.ren * (1 + C_sutherland / p.Tref);
   }
+  a.gm2 = p.gamma * p.Ma * p.Ma;
+  a.kheat = p.gamma / (p.Pr * (p.gamma - 1.0));
   const bool f32 = p.precision == CLES_PREC_FP32BULK;
   a.rho_ref = f32 ? c->ref_rho : 0.0;
   a.p_ref = f32 ? c->ref_p : 0.0;

@@ -5,7 +5,11 @@ restatement.
 Classification (Nvert) and the stencils -- every integer (node, triangle,
 ICP mode, donor indices) and every floating-point field (nearest point,
 distances, weights) -- must be bit-identical; the state after time steps
-within the FP64 bar of parity_util.
+within the FP64 bar of parity_util.  The surface forces of a given state are
+bitwise (zero steps); after steps the inputs differ at round-off, and each
+output vector (F, Cp, A, A_tot, M, Mdpdn, Ia) is held to 1e-12 of its
+magnitude or the oracle's round-off envelope (the moment about the centre of
+a nearly symmetric body cancels).
 """
 import numpy as np
 import pytest
@@ -13,12 +17,49 @@ import pytest
 import chimera_comp_les_amd as cl
 import pyoracle
 from chimera_comp_les_amd import cases
-from parity_util import STATE_TOL, state_err
+from parity_util import ENVELOPE_FACTOR, STATE_TOL, ULP_NOISE, state_err
 
 pytestmark = pytest.mark.gpu
 
 INT_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
 FP_FIELDS = ("d_s", "d_i", "cs1", "cs2", "cs3", "cr1", "cr2", "cr3")
+
+
+FORCE_GROUPS = ((0, 3), (3, 6), (6, 9), (9, 10), (10, 13), (13, 16), (16, 19))
+CENTER = (1.2, 1.2, 1.4)
+
+
+def _group_err(a, b):
+    return np.array([np.linalg.norm(a[i:j] - b[i:j]) / max(np.linalg.norm(b[i:j]), 1e-300) for i, j in FORCE_GROUPS])
+
+
+def oracle_forces(cfg, coor, Q, V, T, steps_list):
+    """oracle forces after each number of steps in steps_list, and their
+    round-off envelope per output vector (one-ulp perturbations of Q)"""
+    def run(Qi):
+        o = pyoracle.Oracle(cfg, coor)
+        o.set_state(Qi)
+        o.ibm_search(V, T)
+        out, done = [], 0
+        for n in steps_list:
+            o.step(n - done)
+            done = n
+            out.append(o.ibm_forces(CENTER))
+        o.close()
+        return out
+    base = run(Q)
+    env = [np.zeros(len(FORCE_GROUPS)) for _ in steps_list]
+    for seed in (11, 12):
+        rng = np.random.default_rng(seed)
+        r = run(Q * (1.0 + ULP_NOISE * rng.standard_normal(Q.shape)))
+        env = [np.maximum(e, _group_err(x, y)) for e, x, y in zip(env, r, base)]
+    return base, env
+
+
+def assert_forces(fg, fo, env):
+    tol = np.maximum(1e-12, ENVELOPE_FACTOR * env)
+    err = _group_err(fg, fo)
+    assert np.all(err <= tol), np.c_[err, tol]
 
 
 def sphere(iso=2, les=0, level=2, jitter=1e-3, twoD=0):
@@ -91,20 +132,22 @@ def test_surface_forces_sphere(iso, les, steps):
     yet, pressure only)."""
     cfg, coor, Q, V, T = sphere(iso=iso, les=les)
     o, g = compare_search(cfg, coor, Q, V, T)
+    o.close()
+    fos, env = oracle_forces(cfg, coor, Q, V, T, (steps, steps + 1))
     if steps:
-        o.step(steps)
         g.advance(steps)
-    center = (1.2, 1.2, 1.4)
-    fo = o.ibm_forces(center)
-    fg = g.ibm_forces(center)
+    fg = g.ibm_forces(CENTER)
+    fo = fos[0]
     assert np.all(np.isfinite(fo)) and (steps == 0 or np.any(fo[0:3] != 0))
-    assert np.array_equal(fg, fo), np.c_[fg, fo, fg - fo]
+    if steps == 0:  # the same state: the gather and the ordered fold are bitwise
+        assert np.array_equal(fg, fo), np.c_[fg, fo, fg - fo]
+    else:
+        assert_forces(fg, fo, env[0])
     # once per step, as Struc_Solver calls it: the work buffers are reused
-    o.step(1)
     g.advance(1)
-    fo, fg = o.ibm_forces(center), g.ibm_forces(center)
+    fg = g.ibm_forces(CENTER)
     g.close()
-    assert np.array_equal(fg, fo), np.c_[fg, fo, fg - fo]
+    assert_forces(fg, fos[1], env[1])
 
 
 @pytest.mark.parametrize("twoD", [1, 3])
@@ -119,9 +162,9 @@ def test_steps_and_forces_sphere_two_d(twoD):
     g.advance(2)
     Qc, Pc = o.state()
     Qg, Pg = g.state()
-    center = (1.2, 1.2, 1.4)
-    fo, fg = o.ibm_forces(center), g.ibm_forces(center)
+    fos, env = oracle_forces(cfg, coor, Q, V, T, (2,))
+    fg = g.ibm_forces(CENTER)
     g.close()
     assert state_err(Qg, Qc) <= STATE_TOL
     assert state_err(Pg[..., None], Pc[..., None]) <= STATE_TOL
-    assert np.array_equal(fg, fo), np.c_[fg, fo]
+    assert_forces(fg, fos[0], env[0])
