@@ -200,17 +200,27 @@ int rk_coef(const cles_params &p, int istage, double coef[3]) {
 int get_coor(cles_ctx *c, double *xyz) { return download(c, c->coor, xyz, 3, CLES_LAYOUT_GLOBAL); }
 
 // device bytes cles_create allocates for p (the allocation list of cles_create)
+// doubles that hold n values of the LES model's fields: double in the FP64
+// form, float in the FP32-bulk form
+size_t les_field_doubles(const cles_params *p, size_t n) {
+  return p->precision == CLES_PREC_FP32BULK ? (n + 1) / 2 : n;
+}
+
 size_t ctx_footprint(const cles_params *p) {
   const long long N = (long long)p->mx * p->my * p->mz;
   const size_t np = (size_t)((N + 63) / 64 * 64);
   size_t planes = 3 + 9 + 1 + 1 + 1 + 15 + 3 + 5;  // coor, metrics, aj, nvert, fil2, Q x3, P x3, rhs
+  size_t extra = 0;  // doubles outside whole planes
   if (p->les) {
-    planes += 1 + 3 + 3;  // tfil2, cs, cs_alt, nut, ua
-    if (p->les != 1 && p->les != 10) planes += 7 + 2 + 3 + 1;  // strain rates, LM, MM, LM/27, MM/27, 1/aj, |S|
+    planes += 1 + 3;  // tfil2, cs, cs_alt, nut
+    if (p->les != 1 && p->les != 10) {
+      planes += 1;  // 1/aj
+      extra += les_field_doubles(p, 7 * np) + 3 * les_field_doubles(p, np);  // strain rates + |S|, LM, MM, |S| of nu_t
+    }
   }
   if (p->averaging) planes += 12;  // Do_averaging sums
   const size_t xfer = (size_t)((p->mx + 4) * (long long)(p->my + 4) * (p->mz + 4)) * 5;
-  return (planes * np + xfer) * sizeof(double);
+  return (planes * np + extra + xfer) * sizeof(double);
 }
 }  // namespace cles
 
@@ -285,15 +295,13 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
     TRY(dalloc(c, &c->cs, np));
     TRY(dalloc(c, &c->cs_alt, np));
     TRY(dalloc(c, &c->nut, np));
-    TRY(dalloc(c, &c->ua, 3 * np));
     if (p->les != 1 && p->les != 10) {
-      TRY(dalloc(c, &c->grad, 7 * np));
-      TRY(dalloc(c, &c->lm, np));
-      TRY(dalloc(c, &c->mm, np));
-      TRY(dalloc(c, &c->lmq, np));
-      TRY(dalloc(c, &c->mmq, np));
+      // the model's own fields in the precision of the form (les_field_doubles)
+      TRY(dalloc(c, &c->grad, les_field_doubles(p, 7 * np)));
+      TRY(dalloc(c, &c->lm, les_field_doubles(p, np)));
+      TRY(dalloc(c, &c->mm, les_field_doubles(p, np)));
+      TRY(dalloc(c, &c->sabs, les_field_doubles(p, np)));
       TRY(dalloc(c, &c->ivol, np));
-      TRY(dalloc(c, &c->sabs, np));
     }
   }
   if (p->averaging) TRY(dalloc(c, &c->avg, 12 * np));
@@ -316,7 +324,7 @@ void cles_destroy(cles_ctx *c) {
   if (!c) return;
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->ua, c->qsnap, c->tfil2, c->lmq, c->mmq, c->ivol, c->sabs, c->avg};
+                    c->fil2, c->qsnap, c->tfil2, c->ivol, c->sabs, c->avg};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   ibm_free(c);

@@ -202,10 +202,10 @@ struct cles_ctx {
   size_t xfer_elems;
   // LES
   double *cs, *cs_alt, *nut;
+  // fields of the dynamic model in the precision of the form (double, or
+  // float for FP32-bulk; the pointers then address float planes of np)
   double *grad;    // 7 planes: strain-rate components S11 S12 S13 S22 S23 S33, |S|
-  double *ua;      // 3 planes: velocity held by lUcat after the gradient loop
   double *lm, *mm;
-  double *lmq, *mmq;  // LM/27, MM/27: the terms of the 27-point means
   double *sabs;       // |S| of the velocity after the gradient loop, for nu_t
   double *ivol;       // 1/aj, the smoothing weights
   cles::SigmaSpec sig;
