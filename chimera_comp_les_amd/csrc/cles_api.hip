@@ -126,7 +126,6 @@ int check_params(const cles_params *p) {
       return set_error(nullptr, CLES_EINVAL, "bctype[%d]=%d is not on this path", q, p->bctype[q]);
   if (p->fluxorder < 1 || p->fluxorder > 3) return set_error(nullptr, CLES_EINVAL, "fluxorder must be 1, 2 or 3");
   if (p->eno_order != 2 && p->eno_order != 3) return set_error(nullptr, CLES_EINVAL, "ENOorder must be 2 or 3");
-  if (p->calc_sigma) return set_error(nullptr, CLES_EINVAL, "-calc_sigma (shock sensor) is not on this path");
   if (p->twoD < 0 || p->twoD > 3) return set_error(nullptr, CLES_EINVAL, "-twoD must be 0, 1, 2 or 3");
   if (p->precision != CLES_PREC_FP64 && p->precision != CLES_PREC_FP32BULK)
     return set_error(nullptr, CLES_EINVAL, "unknown precision %d", p->precision);
@@ -213,6 +212,7 @@ size_t ctx_footprint(const cles_params *p) {
   size_t extra = 0;  // doubles outside whole planes
   if (p->les) {
     planes += 1 + 3;  // tfil2, cs, cs_alt, nut
+    if (p->calc_sigma) planes += 6;  // shock sensor and its smoothing buffer
     if (p->les != 1 && p->les != 10) {
       planes += 1;  // 1/aj
       extra += les_field_doubles(p, 7 * np) + 3 * les_field_doubles(p, np);  // strain rates + |S|, LM, MM, |S| of nu_t
@@ -303,6 +303,10 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
       TRY(dalloc(c, &c->sabs, les_field_doubles(p, np)));
       TRY(dalloc(c, &c->ivol, np));
     }
+    if (p->calc_sigma) {
+      TRY(dalloc(c, &c->sgm, 3 * np));
+      TRY(dalloc(c, &c->sgm_tmp, 3 * np));
+    }
   }
   if (p->averaging) TRY(dalloc(c, &c->avg, 12 * np));
   {
@@ -324,7 +328,7 @@ void cles_destroy(cles_ctx *c) {
   if (!c) return;
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->qsnap, c->tfil2, c->ivol, c->sabs, c->avg};
+                    c->fil2, c->qsnap, c->tfil2, c->ivol, c->sabs, c->avg, c->sgm, c->sgm_tmp};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   ibm_free(c);
@@ -439,7 +443,9 @@ int cles_get_les(cles_ctx *c, double *cs, double *nu_t, double *sigma) {
       memset(nu_t, 0, N * sizeof(double));
     }
   }
-  if (sigma) {
+  if (sigma && c->sgm) {
+    if ((r = download(c, c->sgm, sigma, 3, CLES_LAYOUT_GLOBAL))) return r;
+  } else if (sigma) {
     for (int k = 0; k < c->g.mz; k++)
       for (int j = 0; j < c->g.my; j++)
         for (int i = 0; i < c->g.mx; i++) {
@@ -466,6 +472,7 @@ int cles_form_rhs(cles_ctx *c, int istage, double *rhs) {
   CLES_HIP(c, hipMemsetAsync(c->rhs, 0, 5 * c->g.np * sizeof(double), c->stream));
   int r = reset_norms(c);
   if (r) return r;
+  if (c->sgm && istage == 0 && (r = launch_sigma(c, c->cur))) return r;  // formConvectionLES, rhs.c:3327
   r = launch_stage(c, c->cur, -1, istage, 0, coef);
   if (r) return r;
   if (!c->p.invicid) {
@@ -489,6 +496,7 @@ static int runge_kutta_async(cles_ctx *c) {
     rk_coefficients(c->p, s, coef, &alfa);
     int r = reset_norms(c);
     if (r) return r;
+    if (c->sgm && s == 0 && (r = launch_sigma(c, in))) return r;  // formConvectionLES, rhs.c:3327
     r = launch_stage(c, in, out, s, 1, coef);
     if (r) return r;
     r = launch_boundary(c, in, out, s, alfa, coef);

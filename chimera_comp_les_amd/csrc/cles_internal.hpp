@@ -207,6 +207,7 @@ struct cles_ctx {
   double *grad;    // 7 planes: strain-rate components S11 S12 S13 S22 S23 S33, |S|
   double *lm, *mm;
   double *sabs;       // |S| of the velocity after the gradient loop, for nu_t
+  double *sgm, *sgm_tmp;  // -calc_sigma: 3 planes each, the shock sensor (sgm is the one the fluxes read)
   double *ivol;       // 1/aj, the smoothing weights
   cles::SigmaSpec sig;
   // reductions
@@ -227,6 +228,8 @@ int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const
 // bctype b is implemented on side q (0..5 = i-, i+, j-, j+, k-, k+)
 bool supported_bc(int side, int b);
 int launch_les(cles_ctx *c);
+// -calc_sigma: the shock sensor of the state in buffer buf into c->sgm
+int launch_sigma(cles_ctx *c, int buf);
 int launch_averaging(cles_ctx *c);
 // node coordinates, interleaved x y z in the natural order (global layout)
 int get_coor(cles_ctx *c, double *xyz);

@@ -90,6 +90,7 @@ struct StageArgs {
   int suth;
   int twoD;  // -twoD 1: no xi-direction fluxes, u = 0 in CalcUCat, no rhoU residual
   SigmaSpec sig;
+  const double *sgm;  // -calc_sigma: the sensor (3 planes), else null and sig applies
   double a_rhs, a_old, a_cur;
   int istage;
   int ntx, nty, ntz, per_xcd;
@@ -136,6 +137,7 @@ __device__ __forceinline__ bool interior(const Grid &g, int k, int j, int i) {
   return i >= 1 && i <= g.mx - 2 && j >= 1 && j <= g.my - 2 && k >= 1 && k <= g.mz - 2;
 }
 __device__ __forceinline__ double sigma_at(const StageArgs &a, int d, int k, int j, int i) {
+  if (a.sgm) return a.sgm[nid(a.g, k, j, i) + d * a.g.np];
   return interior(a.g, k, j, i) ? a.sig.in[d] : a.sig.bnd[d];
 }
 
@@ -1133,6 +1135,7 @@ void fill_stage_args(cles_ctx *c, int in, int out, int istage, const double coef
   a.suth = p.sutherland;
   a.twoD = p.twoD;
   a.sig = c->sig;
+  a.sgm = c->sgm;
   a.a_rhs = coef[0]; a.a_old = coef[1]; a.a_cur = coef[2];
   a.istage = istage;
   a.norms = c->dnorm;

@@ -145,6 +145,8 @@ double orc_eno_qr(double qm, double q0, double qp, int k);
 double orc_muscl_ql(double qm, double q0, double qp);
 double orc_muscl_qr(double qm, double q0, double qp);
 double orc_minmod(double a, double b);
+/* sigma_cent with the threshold ks (rhs.c:2334-2357) */
+double orc_sigma_cent(double fL, double f0, double fR, double fRR, double ks);
 double orc_fp_2(const double u[4], const double v[4]);
 double orc_fp4(const double u[4]);
 void orc_rusanov(const double ql[5], double pl, const double qr[5], double pr,

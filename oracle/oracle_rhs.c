@@ -348,6 +348,10 @@ static double sigma_cent(double fL, double f0, double fR, double fRR, double ks)
   return 0.5 + 0.5 * tanh(3.0 * (r - rd) / rmax) / tanh(3.0);
 }
 
+double orc_sigma_cent(double fL, double f0, double fR, double fRR, double ks) {
+  return sigma_cent(fL, f0, fR, fRR, ks);
+}
+
 /* Sigma at istage 0, rhs.c:3327-3444 (parity for -calc_sigma: tanh from libm). */
 static void form_sigma(orc_ctx *c) {
   const orc_params *P = &c->p;

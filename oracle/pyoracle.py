@@ -85,6 +85,8 @@ def lib():
             f = getattr(L, name)
             f.restype = C.c_double
             f.argtypes = [C.c_double] * 3 + [C.c_int]
+        L.orc_sigma_cent.restype = C.c_double
+        L.orc_sigma_cent.argtypes = [C.c_double] * 5
         L.orc_minmod.restype = C.c_double
         L.orc_minmod.argtypes = [C.c_double, C.c_double]
         L.orc_fp_2.restype = C.c_double

@@ -70,5 +70,5 @@ def test_create_rejects_unsupported_options_before_device_use():
     assert e == -1 and "fluxorder" in msg
     e, msg = _create_err(les=2, invicid=1)
     assert e == -1 and "LES" in msg
-    e, msg = _create_err(calc_sigma=1)
-    assert e == -1 and "calc_sigma" in msg
+    e, msg = _create_err(twoD=4)
+    assert e == -1 and "twoD" in msg

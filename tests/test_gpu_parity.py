@@ -215,6 +215,32 @@ def test_hybrid_sigma_blend(fluxorder):
     assert state_err(Qg, Qc) <= STATE_TOL
 
 
+@pytest.mark.parametrize("case", ["tgv", "channel"])
+def test_shock_sensor(case):
+    """-calc_sigma: the sensor of formConvectionLES (rhs.c:3327-3444) from the
+    first stage's pressure -- sigma_cent per direction over its index ranges,
+    clipped to [s_min, s], smoothed by the geometric means of the raw
+    neighbours, the curvature test and sigma_y = sigma_z = min -- drives the
+    hybrid flux; the channel also takes -ABC (sigma = 0 beyond k = ABC)."""
+    if case == "tgv":
+        cfg, coor, Q = cases.tgv_case(N=12)
+        Q = perturbed(Q, 13, amp=3e-2)
+        cfg.update(calc_sigma=1, s_x_min=0.2, s_y=0.9)
+    else:
+        cfg, coor, Q = channel_case(nx=8, ny=14, nz=14, bc=(7, 7, 1, 1, 45, 8), les=2)
+        cfg.update(calc_sigma=1, abc=10)
+    o, g = _pair(cfg, coor, Q)
+    e = rhs_err(g.form_rhs(0), o.form_rhs(0))
+    assert e <= RHS_TOL, e
+    so, sg = o.les_fields()[2], g.les_fields()[2]
+    assert field_err(sg, so) <= 1e-12
+    assert so[1:-1, 1:-1, 1:-1].min() < 0.9 * so[1:-1, 1:-1, 1:-1].max()   # the sensor varies
+    o.close()
+    g.close()
+    g, base = _check_les_steps(cfg, coor, Q, 3)
+    g.close()
+
+
 def test_steps_tgv_les_testfilter_ik():
     """-testfilter_ik: test-filter width 5^(1/3) Delta instead of 2 Delta."""
     cfg, coor, Q = cases.tgv_case(N=12)

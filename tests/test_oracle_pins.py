@@ -482,3 +482,72 @@ def test_poiseuille_inflow_profile():
     L1 = -c * (0.0 - 1.0 * c * (0.0 - w)) * 8.0   # aj zet_z = 1/h on a Cartesian grid
     d0 = (GAMMA - 1) * L1 / c ** 2 + 0.5 * 2 * L1 / c ** 2
     assert np.allclose(rho, 1.0 - cfg["dt"] * d0, rtol=1e-13)
+
+
+# ------------------------------------- shock sensor (-calc_sigma, rhs.c:2334-3444)
+def _sig(fL, f0, fR, fRR, ks=1e-5):
+    return pyoracle.lib().orc_sigma_cent(fL, f0, fR, fRR, ks)
+
+
+def test_sigma_cent_known_answers():
+    """compute_ri(fR, f0, fL) = (|2 dL dR| + ks) / (dL^2 + dR^2 + ks) is 1 on
+    linear (and constant) data and tends to 0 at a jump; sigma_cent takes the
+    smaller ratio r of its two three-point stencils and maps it through
+    1/2 + 1/2 tanh(3 (r - 1/2) / max(1/2, r)) / tanh(3): 1/2 + tanh(1.5)/(2 tanh 3)
+    on smooth data, 0 at a jump (r = 0), 1/2 at r = 1/2."""
+    smooth = 0.5 + 0.5 * math.tanh(1.5) / math.tanh(3.0)
+    for d in (0.0, 1e-3, 2.5, -7.0):
+        assert _sig(1.0, 1.0 + d, 1.0 + 2 * d, 1.0 + 3 * d) == pytest.approx(smooth, rel=1e-15)
+    # a jump between the two middle points: both ratios see one flat and one steep side
+    assert _sig(1.0, 1.0, 50.0, 50.0, ks=1e-12) == pytest.approx(0.0, abs=1e-12)
+    # r = 1/2 exactly: dL = 1, dR = 2 + sqrt(3) gives 2 dL dR / (dL^2 + dR^2) = 1/2
+    dR = 2 + math.sqrt(3)
+    assert _sig(0.0, 1.0, 1.0 + dR, 1.0 + 2 * dR, ks=0.0) == pytest.approx(0.5, abs=1e-14)
+    # the threshold ks keeps a flat stencil smooth (0/0 -> ks/ks)
+    assert _sig(2.0, 2.0, 2.0, 2.0) == pytest.approx(smooth, rel=1e-15)
+
+
+@pytest.mark.parametrize("sx", [1.0, 0.5])
+def test_sigma_sensor_on_a_uniform_pressure(sx):
+    """-calc_sigma on a uniform pressure: every stencil is smooth, so the raw
+    sensor is c = 1/2 + tanh(1.5)/(2 tanh 3) (< 1) on the nodes its ranges
+    cover, clipped to s_x; the smoothing multiplies a node by the geometric
+    means of three neighbour pairs, c * c^3 = c^4 deep inside; the curvature
+    test sees a flat field, and sigma_y = sigma_z = their minimum."""
+    n = 16
+    coor, h = cases.cartesian_grid(n, L=2.0)
+    Q = cases.conserved(np.ones(coor.shape[:3]), 0.1 + 0 * coor[..., 0], 0 * coor[..., 0], 0 * coor[..., 0],
+                        2.0 + 0 * coor[..., 0])
+    cfg = _cfg(n, les=2, calc_sigma=1, s_x=sx, s_y=1.0, s_z=1.0)
+    o = pyoracle.Oracle(cfg, coor)
+    o.set_state(Q)
+    o.form_rhs(0)
+    _, _, sg = o.les_fields()
+    c = 0.5 + 0.5 * math.tanh(1.5) / math.tanh(3.0)
+    deep = sg[6:-6, 6:-6, 6:-6]
+    assert np.allclose(deep[..., 0], min(sx, c) ** 4, rtol=1e-14)
+    assert np.allclose(deep[..., 1], c ** 4, rtol=1e-14) and np.allclose(deep[..., 2], c ** 4, rtol=1e-14)
+    # boundary nodes keep (s_x, s_y, s_z)
+    assert np.allclose(sg[0, 0, 0], (sx, 1.0, 1.0))
+    o.close()
+
+
+def test_sigma_sensor_detects_a_pressure_jump():
+    """A pressure step across x = L/2: nodes whose x stencils straddle it get
+    a raw sensor near 0, and the smoothing spreads the small value to their
+    neighbours; away from the jump the smooth value remains."""
+    n = 16
+    coor, h = cases.cartesian_grid(n, L=2.0)
+    x = coor[..., 0]
+    p = np.where(x < 1.0 - 1e-9, 2.0, 20.0)
+    Q = cases.conserved(np.ones(x.shape), 0 * x, 0 * x, 0 * x, p)
+    cfg = _cfg(n, les=2, calc_sigma=1, sigma_thrsh=1e-12)
+    o = pyoracle.Oracle(cfg, coor)
+    o.set_state(Q)
+    o.form_rhs(0)
+    sx = o.les_fields()[2][8, 8, :, 0]
+    c = 0.5 + 0.5 * math.tanh(1.5) / math.tanh(3.0)
+    jump = int(np.argmax(x[8, 8] >= 1.0 - 1e-9))   # first node of the high side
+    assert sx[jump - 1] < 1e-6 and sx[jump] < 1e-6
+    assert sx[4] == pytest.approx(c ** 4, rel=1e-12)
+    o.close()
