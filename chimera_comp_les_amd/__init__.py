@@ -53,6 +53,7 @@ class CParams(C.Structure):
         ("twoD", C.c_int), ("testfilter_ik", C.c_int),
         ("tistart", C.c_int), ("rstart", C.c_int), ("precision", C.c_int),
         ("averaging", C.c_int),
+        ("i_homo_filter", C.c_int), ("j_homo_filter", C.c_int), ("k_homo_filter", C.c_int),
     ]
 
 

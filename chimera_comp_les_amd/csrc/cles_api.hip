@@ -205,6 +205,9 @@ size_t les_field_doubles(const cles_params *p, size_t n) {
   return p->precision == CLES_PREC_FP32BULK ? (n + 1) / 2 : n;
 }
 
+// sums of the homogeneous LM/MM averages: 5 per (k, j) line, 4 per j plane, 2
+size_t homo_doubles(const cles_params *p) { return 5 * (size_t)p->my * p->mz + 4 * (size_t)p->my + 2; }
+
 size_t ctx_footprint(const cles_params *p) {
   const long long N = (long long)p->mx * p->my * p->mz;
   const size_t np = (size_t)((N + 63) / 64 * 64);
@@ -216,6 +219,7 @@ size_t ctx_footprint(const cles_params *p) {
     if (p->les != 1 && p->les != 10) {
       planes += 1;  // 1/aj
       extra += les_field_doubles(p, 7 * np) + 3 * les_field_doubles(p, np);  // strain rates + |S|, LM, MM, |S| of nu_t
+      if (p->i_homo_filter) extra += homo_doubles(p);
     }
   }
   if (p->averaging) planes += 12;  // Do_averaging sums
@@ -302,6 +306,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
       TRY(dalloc(c, &c->mm, les_field_doubles(p, np)));
       TRY(dalloc(c, &c->sabs, les_field_doubles(p, np)));
       TRY(dalloc(c, &c->ivol, np));
+      if (p->i_homo_filter) TRY(dalloc(c, &c->homo, homo_doubles(p)));
     }
     if (p->calc_sigma) {
       TRY(dalloc(c, &c->sgm, 3 * np));
@@ -328,7 +333,7 @@ void cles_destroy(cles_ctx *c) {
   if (!c) return;
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->qsnap, c->tfil2, c->ivol, c->sabs, c->avg, c->sgm, c->sgm_tmp};
+                    c->fil2, c->qsnap, c->tfil2, c->ivol, c->sabs, c->avg, c->sgm, c->sgm_tmp, c->homo};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   ibm_free(c);

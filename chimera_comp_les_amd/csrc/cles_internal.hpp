@@ -207,6 +207,7 @@ struct cles_ctx {
   double *grad;    // 7 planes: strain-rate components S11 S12 S13 S22 S23 S33, |S|
   double *lm, *mm;
   double *sabs;       // |S| of the velocity after the gradient loop, for nu_t
+  double *homo;           // -i_homo_filter: line, plane and global sums of LM and MM
   double *sgm, *sgm_tmp;  // -calc_sigma: 3 planes each, the shock sensor (sgm is the one the fluxes read)
   double *ivol;       // 1/aj, the smoothing weights
   cles::SigmaSpec sig;

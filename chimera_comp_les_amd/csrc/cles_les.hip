@@ -50,6 +50,8 @@ struct LesArgs {
   int les, testfilter_ik;
   int twoD;  // 1: the last CalcUCat set u = 0 (rhs.c:1349); the loop's own velocities keep it
   double max_cs;
+  int ih, jh, kh;  // -i/j/k_homo_filter
+  double *homo;    // homogeneous sums: 5 per (k, j) line, 3 per j plane, 2 global
 };
 
 template <typename T> __device__ __forceinline__ double nvat(const LesArgs<T> &a, long long n) {
@@ -379,6 +381,102 @@ __device__ __forceinline__ bool halo_node(const Grid &g, const NodeTile &T, int 
   return ii >= 0 && jj >= 0 && kk >= 0 && ii < g.mx && jj < g.my && kk < g.mz;
 }
 
+// Homogeneous-direction averages of LM and MM (les.c:934-1116).  One thread
+// per (k, j) line sums along i (the fluid nodes, with their count, and every
+// node), one thread per j plane sums its lines over k, one thread the planes.
+// homo layout: line sums [5][mz][my], plane means/sums [3][my], global [2].
+template <typename T>
+__global__ __launch_bounds__(256) void homo_line_kernel(LesArgs<T> a) {
+  const Grid &g = a.g;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L = (long long)g.my * g.mz;
+  if (t >= L) return;
+  const int j = (int)(t % g.my), k = (int)(t / g.my);
+  double sl = 0, sm = 0, cnt = 0, al = 0, am = 0;
+  if (k >= 1 && k <= g.mz - 2 && j >= 1 && j <= g.my - 2) {
+    for (int i = 1; i <= g.mx - 2; i++) {
+      const long long n = nid(g, k, j, i);
+      const double l = double(a.lm[n]), m = double(a.mm[n]);
+      if (nvat(a, n) < 0.1) {
+        sl += l;
+        sm += m;
+        cnt += 1;
+      }
+      al += l;
+      am += m;
+    }
+  }
+  double *h = a.homo;
+  h[t] = sl;
+  h[L + t] = sm;
+  h[2 * L + t] = cnt;
+  h[3 * L + t] = al;
+  h[4 * L + t] = am;
+}
+template <typename T>
+__global__ __launch_bounds__(256) void homo_plane_kernel(LesArgs<T> a) {
+  const Grid &g = a.g;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.my) return;
+  const long long L = (long long)g.my * g.mz;
+  const double *h = a.homo;
+  double sl = 0, sm = 0, cnt = 0, al = 0, am = 0;
+  for (int k = 1; k <= g.mz - 2; k++) {
+    const long long t = (long long)k * g.my + j;
+    sl += h[t];
+    sm += h[L + t];
+    cnt += h[2 * L + t];
+    al += h[3 * L + t];
+    am += h[4 * L + t];
+  }
+  double *P = a.homo + 5 * L;
+  if (cnt > 0) {
+    sl /= cnt;
+    sm /= cnt;
+  }
+  P[j] = sl;                  // <LM>_j over the fluid nodes of plane j
+  P[g.my + j] = sm;           // <MM>_j
+  P[2 * g.my + j] = al;       // sum of LM over the plane's interior
+  P[3 * g.my + j] = am;
+}
+template <typename T>
+__global__ void homo_global_kernel(LesArgs<T> a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const Grid &g = a.g;
+  const long long L = (long long)g.my * g.mz;
+  const double *P = a.homo + 5 * L;
+  double al = 0, am = 0;
+  for (int j = 1; j <= g.my - 2; j++) {
+    al += P[2 * g.my + j];
+    am += P[3 * g.my + j];
+  }
+  a.homo[5 * L + 4 * g.my] = al;
+  a.homo[5 * L + 4 * g.my + 1] = am;
+}
+// Cs after the homogeneous averages of the reference, in its order: the
+// (i, k) plane mean on interior nodes, the i-line mean on interior fluid
+// nodes, the global ratio on every node
+template <typename T>
+__device__ __forceinline__ double homogeneous_cs(const LesArgs<T> &a, int k, int j, int i, double nv, double cs) {
+  const Grid &g = a.g;
+  const long long L = (long long)g.my * g.mz;
+  const bool in = interior(g, k, j, i);
+  const double *P = a.homo + 5 * L;
+  if (a.ih && a.kh && in) cs = 0.5 * P[j] / (P[g.my + j] + LES_EPS);
+  if (a.ih && !a.kh && in && nv < 0.1) {
+    const long long t = (long long)k * g.my + j;
+    double sl = a.homo[t], sm = a.homo[L + t];
+    const double cnt = a.homo[2 * L + t];
+    if (cnt > 0) {
+      sl /= cnt;
+      sm /= cnt;
+    }
+    cs = 0.5 * sl / (sm + LES_EPS);
+  }
+  if (a.ih && a.jh && a.kh) cs = P[4 * g.my] / P[4 * g.my + 1];
+  return cs;
+}
+
 // <LM>/<MM> (integrate_t: the 27-point mean, les.c:66-78) and clipping on
 // every owned node (les.c:889-930, 1146-1161)
 template <typename T>
@@ -388,7 +486,8 @@ __global__ __launch_bounds__(256) void les_cs_kernel(LesArgs<T> a, int ntx, int 
   const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
   if (tile >= ntiles) return;
   const NodeTile Tn = node_tile(g, tile, ntx, nty);
-  if (a.les != 3) {
+  const bool homo = a.ih || a.jh || a.kh;
+  if (a.les != 3 && !homo) {
     for (int w = threadIdx.x; w < MNB; w += 256) {
       int kk, jj, ii;
       if (!halo_node(g, Tn, w, kk, jj, ii)) continue;
@@ -408,7 +507,7 @@ __global__ __launch_bounds__(256) void les_cs_kernel(LesArgs<T> a, int ntx, int 
       cs = 0;
     } else {
       T LM_avg = 0, MM_avg = 0;
-      if (a.les == 3) {
+      if (a.les == 3 || homo) {
         LM_avg = a.lm[n];
         MM_avg = a.mm[n];
       } else {
@@ -429,6 +528,7 @@ __global__ __launch_bounds__(256) void les_cs_kernel(LesArgs<T> a, int ntx, int 
       cs = a.les == 1 ? 0.01 : (C < 0.0001 ? 0.0001 : C);
     }
   }
+  if (homo) cs = homogeneous_cs(a, k, j, i, nv, cs);
   if (nv > 1.1 && nv < 7.) {
     cs = 0.0;
   } else {
@@ -468,7 +568,7 @@ __global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs<T> a, int ntx, 
   const bool in = interior(g, Tn.k, Tn.j, Tn.i);
   const double nv = nvat(a, n);
   double cs;
-  if (!in || !(nv < 0.1) || a.les == 3) {
+  if (!in || !(nv < 0.1) || a.les == 3 || a.ih || a.jh || a.kh) {
     cs = a.cs[n];
   } else {
     double vs = 0.0, ws = 0.0;
@@ -531,6 +631,8 @@ int launch_les_t(cles_ctx *c) {
   a.ivol = c->ivol; a.fil2 = c->fil2; a.tfil2 = c->tfil2;
   a.cs = c->cs; a.cs_out = c->cs_alt; a.nut = c->nut;
   a.les = p.les; a.testfilter_ik = p.testfilter_ik; a.max_cs = p.max_cs; a.twoD = p.twoD;
+  a.ih = p.i_homo_filter; a.jh = p.j_homo_filter; a.kh = p.k_homo_filter;
+  a.homo = c->homo;
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
   const int bs = 256;
   const unsigned gN = (unsigned)((N + bs - 1) / bs);
@@ -563,6 +665,15 @@ int launch_les_t(cles_ctx *c) {
     hipLaunchKernelGGL(les_filter_kernel<T>, dim3(8 * pxf), dim3(256), 0, c->stream, a, ntx, nty, ntiles, pxf);
   }
   CLES_HIP(c, hipGetLastError());
+  if (a.ih) {  // the averaged branches (-j or -k alone average nothing)
+    const long long L = (long long)c->g.my * c->g.mz;
+    hipLaunchKernelGGL(homo_line_kernel<T>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(homo_plane_kernel<T>, dim3((unsigned)((c->g.my + 255) / 256)), dim3(256), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(homo_global_kernel<T>, dim3(1), dim3(64), 0, c->stream, a);
+    CLES_HIP(c, hipGetLastError());
+  }
   {
     const int ntx = (c->g.mx + MTX - 1) / MTX, nty = (c->g.my + MTY - 1) / MTY, ntz = (c->g.mz + MTZ - 1) / MTZ;
     const int ntiles = ntx * nty * ntz, pxm = (ntiles + 7) / 8;

@@ -69,7 +69,7 @@ typedef struct cles_params {
   int central;           /* -central: 2 skew-symmetric 4-point (default)      */
   int sutherland;        /* -visT                                             */
   int isothermal;        /* -isothermal                                       */
-  int calc_sigma;        /* -calc_sigma (must be 0 on this path)              */
+  int calc_sigma;        /* -calc_sigma: shock sensor of the hybrid LES flux  */
   int abc;               /* -ABC                                              */
   int twoD;              /* -twoD: 0, or 1/2/3 = drop x/y/z (1: no xi fluxes,
                           * u = 0, no rhoU residual, rhs.c:1349, 1487, 2135,
@@ -81,6 +81,9 @@ typedef struct cles_params {
   int rstart;            /* restart flag                                      */
   int precision;         /* CLES_PREC_*                                       */
   int averaging;         /* -averaging (main.c:779): Do_averaging after steps */
+  int i_homo_filter;     /* -i_homo_filter -j_homo_filter -k_homo_filter      */
+  int j_homo_filter;     /* (main.c:791-793): Cs from LM/MM averaged over the  */
+  int k_homo_filter;     /* homogeneous directions (les.c:934-1116)           */
 } cles_params;
 
 typedef struct cles_ctx cles_ctx;

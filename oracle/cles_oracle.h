@@ -64,6 +64,7 @@ typedef struct {
   int testfilter_ik;       /* -testfilter_ik (main.c:67,788)                 */
   int tistart;             /* time step the run starts from (main.c:890)     */
   int rstart;              /* restart flag (main.c:810)                      */
+  int i_homo_filter, j_homo_filter, k_homo_filter; /* -i/j/k_homo_filter (main.c:791-793) */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;

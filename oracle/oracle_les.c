@@ -179,9 +179,64 @@ static double integ_t(const double v[27]) { /* integrate_t, les.c:66-78 */
   return s;
 }
 
-/* Compute_Smagorinsky_Constant_1, les.c:135-1472 (3-D box filter branch;
- * the homogeneous-direction averages need -i/j/k_homo_filter and are not
- * restated). */
+/* Homogeneous-direction averages of LM and MM (les.c:934-1116), after the
+ * pointwise Cs: with -i and -k_homo_filter the mean over each j plane of the
+ * interior fluid nodes, Cs = 0.5 <LM>_j / (<MM>_j + eps) on every interior
+ * node; with -i alone the mean along each (k, j) line, set on the interior
+ * fluid nodes; with all three, Cs = sum LM / sum MM over the interior (no
+ * fluid test) on every owned node.  -j or -k alone average nothing (their
+ * branches are commented out in the reference) but, like the others, drop
+ * the 27-point means and the smoothing. */
+static void homogeneous_cs(orc_ctx *c, const double *LM, const double *MM) {
+  const orc_params *P = &c->p;
+  const int mx = c->mx, my = c->my, mz = c->mz;
+  if (P->i_homo_filter && P->k_homo_filter) {
+    double *JL = calloc(my, sizeof(double)), *JM = calloc(my, sizeof(double));
+    int *cnt = calloc(my, sizeof(int));
+    for (int k = 1; k < mz - 1; k++)
+      for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++)
+          if (c->lnvert[LI(c, k, j, i)] < 0.1) {
+            JL[j] += LM[LI(c, k, j, i)];
+            JM[j] += MM[LI(c, k, j, i)];
+            cnt[j]++;
+          }
+    for (int j = 0; j < my; j++)
+      if (cnt[j] > 0) { JL[j] /= (double)cnt[j]; JM[j] /= (double)cnt[j]; }
+    for (int j = 1; j < my - 1; j++)
+      for (int k = 1; k < mz - 1; k++)
+        for (int i = 1; i < mx - 1; i++) c->CS[GI(c, k, j, i)] = 0.5 * JL[j] / (JM[j] + LES_EPS);
+    free(JL); free(JM); free(cnt);
+  }
+  if (P->i_homo_filter && !P->k_homo_filter) {
+    for (int k = 1; k < mz - 1; k++)
+      for (int j = 1; j < my - 1; j++) {
+        double sl = 0.0, sm = 0.0;
+        int cnt = 0;
+        for (int i = 1; i < mx - 1; i++)
+          if (c->lnvert[LI(c, k, j, i)] < 0.1) {
+            sl += LM[LI(c, k, j, i)];
+            sm += MM[LI(c, k, j, i)];
+            cnt++;
+          }
+        if (cnt > 0) { sl /= (double)cnt; sm /= (double)cnt; }
+        for (int i = 1; i < mx - 1; i++)
+          if (c->lnvert[LI(c, k, j, i)] < 0.1) c->CS[GI(c, k, j, i)] = 0.5 * sl / (sm + LES_EPS);
+      }
+  }
+  if (P->i_homo_filter && P->j_homo_filter && P->k_homo_filter) {
+    double sl = 0.0, sm = 0.0;
+    for (int k = 1; k < mz - 1; k++)
+      for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++) {
+          sm += MM[LI(c, k, j, i)];
+          sl += LM[LI(c, k, j, i)];
+        }
+    for (size_t n = 0; n < c->nG; n++) c->CS[n] = sl / sm;
+  }
+}
+
+/* Compute_Smagorinsky_Constant_1, les.c:135-1472. */
 static void smagorinsky(orc_ctx *c) {
   const orc_params *P = &c->p;
   const int mx = c->mx, my = c->my, mz = c->mz;
@@ -198,6 +253,7 @@ static void smagorinsky(orc_ctx *c) {
     return;
   }
   const double max_cs = P->max_cs;
+  const int homo = P->i_homo_filter || P->j_homo_filter || P->k_homo_filter;
   orc_v3 *Ax = calloc(c->nL, sizeof(orc_v3)), *Ay = calloc(c->nL, sizeof(orc_v3)), *Az = calloc(c->nL, sizeof(orc_v3));
   double *S = calloc(c->nL, sizeof(double)), *LM = calloc(c->nL, sizeof(double)), *MM = calloc(c->nL, sizeof(double));
   gradients_two_array(c, Ax, Ay, Az, S, 1);
@@ -299,11 +355,12 @@ static void smagorinsky(orc_ctx *c) {
               LM0[idx] = LM[n]; MM0[idx] = MM[n];
             }
         double LM_avg, MM_avg;
-        if (P->les == 3) { LM_avg = LM[LI(c, k, j, i)]; MM_avg = MM[LI(c, k, j, i)]; }
+        if (homo || P->les == 3) { LM_avg = LM[LI(c, k, j, i)]; MM_avg = MM[LI(c, k, j, i)]; }
         else { LM_avg = integ_t(LM0); MM_avg = integ_t(MM0); }
         double C = LM_avg / (MM_avg + LES_EPS);
         c->CS[g] = P->les == 1 ? 0.01 : pmax(C, 0.0001);
       }
+  homogeneous_cs(c, LM, MM);
   /* clipping on every owned node (les.c:1146-1161); the bctype clauses of
    * that test are constant-false ("!bctype==7") */
   for (int k = 0; k < mz; k++)
@@ -320,7 +377,7 @@ static void smagorinsky(orc_ctx *c) {
       }
   orc_g2l(c, c->CS, c->lCS, 1);
   seam_copy_local(c, c->lCS, 1);
-  if (P->les != 3) {
+  if (P->les != 3 && !homo) {
     /* volume-weighted 27-point smoothing of Cs (les.c:1258-1337) */
     double *tmp = malloc(c->nL * sizeof(double));
     memcpy(tmp, c->lCS, c->nL * sizeof(double));

@@ -32,6 +32,7 @@ class OrcParams(C.Structure):
         ("isothermal", C.c_int), ("calc_sigma", C.c_int), ("abc", C.c_int),
         ("twoD", C.c_int), ("testfilter_ik", C.c_int),
         ("tistart", C.c_int), ("rstart", C.c_int),
+        ("i_homo_filter", C.c_int), ("j_homo_filter", C.c_int), ("k_homo_filter", C.c_int),
     ]
 
 
@@ -130,7 +131,8 @@ def make_params(cfg):
                     max_cs=0.05, Twall=1.0, s_x=1.0, s_y=1.0, s_z=1.0, s_x_min=0.0,
                     s_y_min=0.0, s_z_min=0.0, sigma_thrsh=1e-5, ratio=0.0, les=0, invicid=0,
                     fluxorder=3, eno_order=2, central=2, sutherland=0, isothermal=0,
-                    calc_sigma=0, abc=0, twoD=0, testfilter_ik=0, tistart=0, rstart=0)
+                    calc_sigma=0, abc=0, twoD=0, testfilter_ik=0, tistart=0, rstart=0,
+                    i_homo_filter=0, j_homo_filter=0, k_homo_filter=0)
     d = dict(defaults)
     d.update({k: v for k, v in cfg.items() if k in defaults})
     p.mx, p.my, p.mz = cfg["mx"], cfg["my"], cfg["mz"]

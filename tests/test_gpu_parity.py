@@ -241,6 +241,21 @@ def test_shock_sensor(case):
     g.close()
 
 
+@pytest.mark.parametrize("flags", [(1, 0, 1), (1, 0, 0), (1, 1, 1), (0, 1, 0)])
+def test_homogeneous_filters(flags):
+    """-i/j/k_homo_filter: Cs from LM and MM averaged over (i, k) planes,
+    i lines or the whole domain (les.c:934-1116), no smoothing; on a channel
+    (walls in j, the planes differ) and steps as in the other LES tests."""
+    ih, jh, kh = flags
+    cfg, coor, Q = channel_case(nx=10, ny=12, nz=10, bc=(7, 7, 1, 1, 45, 8), les=2)
+    cfg.update(i_homo_filter=ih, j_homo_filter=jh, k_homo_filter=kh)
+    g, base = _check_les_steps(cfg, coor, Q, 3)
+    g.close()
+    cs = base[4]
+    if ih and kh and not jh:   # one value per j plane
+        assert np.ptp(cs[1:-1, 1:-1, 1:-1], axis=(0, 2)).max() == 0.0
+
+
 def test_steps_tgv_les_testfilter_ik():
     """-testfilter_ik: test-filter width 5^(1/3) Delta instead of 2 Delta."""
     cfg, coor, Q = cases.tgv_case(N=12)

@@ -551,3 +551,66 @@ def test_sigma_sensor_detects_a_pressure_jump():
     assert sx[jump - 1] < 1e-6 and sx[jump] < 1e-6
     assert sx[4] == pytest.approx(c ** 4, rel=1e-12)
     o.close()
+
+
+# ------------------------- homogeneous-direction Cs (les.c:934-1116)
+def _homo_case(**flags):
+    """Taylor-Green flow perturbed so that LM and MM vary in every direction"""
+    cfg, coor, Q = cases.tgv_case(N=10)
+    rng = np.random.default_rng(21)
+    Q = cases.fill_seams(Q * (1 + 3e-2 * rng.standard_normal(Q.shape[:3] + (1,))))
+    cfg = dict(cfg, max_cs=10.0, **flags)
+    o = pyoracle.Oracle(cfg, coor)
+    o.set_state(Q)
+    o.form_rhs(0)
+    o.ti = 5
+    lm, mm = o.les_update_debug()
+    cs, _, _ = o.les_fields()
+    o.close()
+    return lm, mm, cs
+
+
+def _clip(c, max_cs=10.0):
+    return np.minimum(np.maximum(c, 0.0), max_cs)
+
+
+def test_homogeneous_planes_i_and_k():
+    """-i_homo_filter -k_homo_filter: Cs on every interior node of plane j is
+    0.5 <LM>_j / (<MM>_j + 1e-6), the means over that plane's interior fluid
+    nodes -- no 27-point means and no smoothing."""
+    lm, mm, cs = _homo_case(i_homo_filter=1, k_homo_filter=1)
+    inner = (slice(1, -1),) * 3
+    L = lm[inner].mean(axis=(0, 2))
+    M = mm[inner].mean(axis=(0, 2))
+    want = _clip(0.5 * L / (M + 1e-6))
+    got = cs[inner]
+    assert np.allclose(got, want[None, :, None], rtol=1e-12, atol=1e-15)
+    assert np.ptp(want) > 0   # the planes differ
+
+
+def test_homogeneous_lines_i():
+    """-i_homo_filter alone: the means along each (k, j) line of i."""
+    lm, mm, cs = _homo_case(i_homo_filter=1)
+    inner = (slice(1, -1),) * 3
+    L = lm[inner].mean(axis=2)
+    M = mm[inner].mean(axis=2)
+    want = _clip(0.5 * L / (M + 1e-6))
+    assert np.allclose(cs[inner], want[:, :, None], rtol=1e-12, atol=1e-15)
+
+
+def test_homogeneous_global():
+    """all three: Cs = sum LM / sum MM over the interior on every owned node
+    (no 0.5, no eps), then the clipping."""
+    lm, mm, cs = _homo_case(i_homo_filter=1, j_homo_filter=1, k_homo_filter=1)
+    inner = (slice(1, -1),) * 3
+    want = _clip(lm[inner].sum() / mm[inner].sum())
+    assert np.allclose(cs, want, rtol=1e-12)
+
+
+def test_homogeneous_j_alone_is_pointwise():
+    """-j_homo_filter alone: no average (the reference's j/k-only branches are
+    commented out), Cs = max(LM / (MM + 1e-6), 1e-4) node by node, unsmoothed."""
+    lm, mm, cs = _homo_case(j_homo_filter=1)
+    inner = (slice(1, -1),) * 3
+    want = _clip(np.maximum(lm[inner] / (mm[inner] + 1e-6), 1e-4))
+    assert np.allclose(cs[inner], want, rtol=1e-12)
