@@ -54,6 +54,7 @@ class CParams(C.Structure):
         ("tistart", C.c_int), ("rstart", C.c_int), ("precision", C.c_int),
         ("averaging", C.c_int),
         ("i_homo_filter", C.c_int), ("j_homo_filter", C.c_int), ("k_homo_filter", C.c_int),
+        ("weno5", C.c_int), ("roe", C.c_int),
     ]
 
 

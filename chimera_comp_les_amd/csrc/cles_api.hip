@@ -130,6 +130,8 @@ int check_params(const cles_params *p) {
   if (p->precision != CLES_PREC_FP64 && p->precision != CLES_PREC_FP32BULK)
     return set_error(nullptr, CLES_EINVAL, "unknown precision %d", p->precision);
   if (p->les && p->invicid) return set_error(nullptr, CLES_EINVAL, "LES requires a viscous run (-inv 0)");
+  if ((p->weno5 || p->roe) && p->precision != CLES_PREC_FP64)
+    return set_error(nullptr, CLES_EINVAL, "the WENO5 and Roe schemes run in the FP64 form only");
   return CLES_OK;
 }
 

@@ -201,6 +201,7 @@ int cles_batch_add(cles_batch *b, cles_ctx *c) {
   if (std::find(b->members.begin(), b->members.end(), c) != b->members.end())
     return batch_error(b, CLES_EINVAL, "context is already a member");
   if (c->p.fluxorder < 2) return batch_error(b, CLES_EINVAL, "batch members run RK2 or RK3 (fluxorder 2 or 3)");
+  if (c->p.weno5 || c->p.roe) return batch_error(b, CLES_EINVAL, "batch members use the reference schemes (no weno5 / roe)");
   if (!b->members.empty() && b->members[0]->p.fluxorder != c->p.fluxorder)
     return batch_error(b, CLES_EINVAL, "batch members must share fluxorder");
   const size_t need = ctx_footprint(&c->p);

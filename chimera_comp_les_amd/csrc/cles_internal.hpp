@@ -99,6 +99,72 @@ template <typename T> __device__ __forceinline__ T weno_qr(T qm, T q0, T qp) {
   T Qr1 = T(0.5) * qm + T(0.5) * q0;
   return weno_mix(Qr0, Qr1, (qp - q0) * (qp - q0), (qm - q0) * (qm - q0));
 }
+// WENO5 of Jiang & Shu (1996): the left state at face i+1/2 from q[i-2..i+2];
+// alpha_k = d_k / (1e-6 + beta_k)^2 normalised with one reciprocal (the
+// weights multiplied through by the product of the squared denominators).
+// No counterpart in the reference: the option weno5.
+__device__ __forceinline__ double weno5_ql(double q0, double q1, double q2, double q3, double q4) {
+  const double eps = 1e-6;
+  const double s0 = q0 - 2 * q1 + q2, t0 = q0 - 4 * q1 + 3 * q2;
+  const double s1 = q1 - 2 * q2 + q3, t1 = q1 - q3;
+  const double s2 = q2 - 2 * q3 + q4, t2 = 3 * q2 - 4 * q3 + q4;
+  const double e0 = eps + (13.0 / 12.0 * s0 * s0 + 0.25 * t0 * t0);
+  const double e1 = eps + (13.0 / 12.0 * s1 * s1 + 0.25 * t1 * t1);
+  const double e2 = eps + (13.0 / 12.0 * s2 * s2 + 0.25 * t2 * t2);
+  const double f0 = e0 * e0, f1 = e1 * e1, f2 = e2 * e2;
+  const double A0 = 0.1 * (f1 * f2), A1 = 0.6 * (f0 * f2), A2 = 0.3 * (f0 * f1);
+  const double p0 = (2 * q0 - 7 * q1 + 11 * q2) * (1.0 / 6.0), p1 = (-q1 + 5 * q2 + 2 * q3) * (1.0 / 6.0),
+               p2 = (2 * q2 + 5 * q3 - q4) * (1.0 / 6.0);
+  return (A0 * p0 + A1 * p1 + A2 * p2) * rcp64(A0 + A1 + A2);
+}
+
+// Roe flux (Roe 1981) across the face metric h in the contravariant form of
+// rusanov(); Roe averages, gamma of the equation of state, Harten's entropy
+// fix (delta = 0.1 c) on the two acoustic waves.  s = {rho, rhoU, rhoV, rhoW,
+// rhoE, p}.  No counterpart in the reference: the option roe.
+__device__ __forceinline__ void roe_flux(const double *L, const double *R, const double *h, double gamma,
+                                         double *F) {
+  const double irL = rcp64(L[0]), irR = rcp64(R[0]);
+  const double uL[3] = {L[1] * irL, L[2] * irL, L[3] * irL}, uR[3] = {R[1] * irR, R[2] * irR, R[3] * irR};
+  const double UL = h[0] * uL[0] + h[1] * uL[1] + h[2] * uL[2], UR = h[0] * uR[0] + h[1] * uR[1] + h[2] * uR[2];
+  const double gii = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+  const double hn = sqrt(gii), ihn = rcp64(hn);
+  const double n[3] = {h[0] * ihn, h[1] * ihn, h[2] * ihn};
+  const double HL = (L[4] + L[5]) * irL, HR = (R[4] + R[5]) * irR;
+  const double sL = sqrt(L[0]), sR = sqrt(R[0]), iws = rcp64(sL + sR);
+  double u[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) u[c] = (sL * uL[c] + sR * uR[c]) * iws;
+  const double H = (sL * HL + sR * HR) * iws, rho = sL * sR;
+  const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const double c2 = (gamma - 1) * (H - 0.5 * q2), c = sqrt(c2), ic2 = rcp64(c2);
+  const double Un = n[0] * u[0] + n[1] * u[1] + n[2] * u[2];
+  const double dr = R[0] - L[0], dp = R[5] - L[5];
+  double du[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) du[k] = uR[k] - uL[k];
+  const double dUn = n[0] * du[0] + n[1] * du[1] + n[2] * du[2];
+  const double a1 = (dp - rho * c * dUn) * (0.5 * ic2), a2 = dr - dp * ic2, a5 = (dp + rho * c * dUn) * (0.5 * ic2);
+  const double delta = 0.1 * c;
+  auto fix = [&](double lam) {
+    const double a = fabs(lam);
+    return a < delta ? (lam * lam + delta * delta) * rcp64(2 * delta) : a;
+  };
+  const double l1 = fix(Un - c), l2 = fabs(Un), l5 = fix(Un + c);
+  double dt[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) dt[k] = du[k] - dUn * n[k];
+  const double udt = u[0] * dt[0] + u[1] * dt[1] + u[2] * dt[2];
+  const double half = 0.5 * hn;
+  F[0] = 0.5 * (L[0] * UL + R[0] * UR) - half * (l1 * a1 + l2 * a2 + l5 * a5);
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    F[1 + k] = 0.5 * ((L[1 + k] * UL + L[5] * h[k]) + (R[1 + k] * UR + R[5] * h[k])) -
+               half * (l1 * a1 * (u[k] - c * n[k]) + l2 * (a2 * u[k] + rho * dt[k]) + l5 * a5 * (u[k] + c * n[k]));
+  F[4] = 0.5 * ((L[4] + L[5]) * UL + (R[4] + R[5]) * UR) -
+         half * (l1 * a1 * (H - c * Un) + l2 * (a2 * 0.5 * q2 + rho * udt) + l5 * a5 * (H + c * Un));
+}
+
 template <typename T> __device__ __forceinline__ T eno_qr(T qm, T q0, T qp, int k) {
   if (k == 2) return T(0.5) * q0 + T(0.5) * qp;
   return -qm * T(0.1666666666666666) + q0 * T(0.8333333333333333) + qp * T(0.3333333333333333);

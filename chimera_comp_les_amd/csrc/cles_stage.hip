@@ -91,6 +91,7 @@ struct StageArgs {
   int twoD;  // -twoD 1: no xi-direction fluxes, u = 0 in CalcUCat, no rhoU residual
   SigmaSpec sig;
   const double *sgm;  // -calc_sigma: the sensor (3 planes), else null and sig applies
+  int weno5, roe;     // build-only schemes (read by the SCH = 1 instantiations only)
   double a_rhs, a_old, a_cur;
   int istage;
   int ntx, nty, ntz, per_xcd;
@@ -261,8 +262,28 @@ __device__ void recon_split(const StageArgs &a, const Sv &sv, bool muscl, long l
   }
 }
 
-// formConvection face (rhs.c:2646-2814, 2817-2971, 2973-3121)
-template <typename T>
+// WENO5 (option weno5, no reference counterpart) where the face's six-node
+// stencil lies in the grid and holds no solid node; the reference's WENO3 /
+// ENO elsewhere
+__device__ __forceinline__ bool weno5_fits(const StageArgs &a, int id, int m, long long n0, long long s) {
+  if (!a.weno5 || id < 2 || id > m - 4) return false;
+  return !(nvat(a, n0 - 2 * s) > 2.9 || nvat(a, n0 - s) > 2.9 || nvat(a, n0 + 2 * s) > 2.9 ||
+           nvat(a, n0 + 3 * s) > 2.9);
+}
+__device__ __forceinline__ void recon_weno5(const StageArgs &a, long long n0, long long s, double *L, double *R) {
+  double v[6][6];
+#pragma unroll
+  for (int t = 0; t < 6; t++) load6(a, n0 + (t - 2) * s, v[t]);
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    L[q] = weno5_ql(v[0][q], v[1][q], v[2][q], v[3][q], v[4][q]);
+    R[q] = weno5_ql(v[5][q], v[4][q], v[3][q], v[2][q], v[1][q]);
+  }
+}
+
+// formConvection face (rhs.c:2646-2814, 2817-2971, 2973-3121).  SCH 1: the
+// build-only schemes of the options weno5 / roe may replace WENO3 / Rusanov.
+template <typename T, int SCH>
 __device__ void conv_face_plain(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]) {
   const Grid &g = a.g;
   const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
@@ -295,12 +316,27 @@ __device__ void conv_face_plain(const StageArgs &a, const Sv &sv, int d, int k, 
     if constexpr (F32) {
       recon_split(a, sv, muscl, nm, n0, n1, n2, eno_left, mask, L, R);
     } else {
-      if (muscl) recon_muscl(a, sv, nm, n0, n1, n2, L, R);
-      else recon_weno(a, sv, nm, n0, n1, n2, eno_left, mask, L, R);
+      bool done = false;
+      if constexpr (SCH != 0) {
+        if (!muscl && mask == 0 && weno5_fits(a, id, m, n0, s)) {
+          recon_weno5(a, n0, s, L, R);
+          done = true;
+        }
+      }
+      if (!done) {
+        if (muscl) recon_muscl(a, sv, nm, n0, n1, n2, L, R);
+        else recon_weno(a, sv, nm, n0, n1, n2, eno_left, mask, L, R);
+      }
     }
   }
   S h[3];
   face_metric(a, d, n0, n1, h);
+  if constexpr (SCH != 0 && !F32) {
+    if (a.roe) {
+      roe_flux(L, R, h, a.gamma, F);
+      return;
+    }
+  }
   if constexpr (F32) {
     const double uconL = (id == m - 2 && a.bc[2 * d + 1] == 1) ? ucon_d(R, h) : ucon_d(L, h);
     const double uconR = (id == 0 && a.bc[2 * d] == 1) ? uconL : ucon_d(R, h);
@@ -315,7 +351,7 @@ __device__ void conv_face_plain(const StageArgs &a, const Sv &sv, int d, int k, 
   }
 }
 
-template <typename T>
+template <typename T, int SCH>
 __device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]);
 
 // FP32-bulk four-point central flux (sigma = 1).  fp_2(c, U) = c * fp4(U) for
@@ -389,12 +425,12 @@ __device__ bool conv_face_les_split(const StageArgs &a, const Sv &sv, int d, int
 // formConvectionLES face (rhs.c:3466-3706, 3714-3937, 3947-4168).  The
 // FP32-bulk form uses the split central flux above and, for the rare faces
 // it does not cover (boundaries, immersed nodes, sigma < 1), this function in
-// double.
-template <typename T>
+// double.  SCH 1: the upwind part may use the weno5 / roe schemes.
+template <typename T, int SCH>
 __device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]) {
   if constexpr (sizeof(T) == 4) {
     // the double fallback reads global memory: the staged velocity is the float form's
-    if (!conv_face_les_split(a, sv, d, k, j, i, F)) conv_face_les<double>(a, Sv{nullptr, 0, 0, 0, false}, d, k, j, i, F);
+    if (!conv_face_les_split(a, sv, d, k, j, i, F)) conv_face_les<double, 0>(a, Sv{nullptr, 0, 0, 0, false}, d, k, j, i, F);
     return;
   }
   const Grid &g = a.g;
@@ -492,18 +528,34 @@ __device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, in
   } else if (a.fluxorder == 2) {
     recon_muscl(a, gv, nm, n0, n1, n2, L, R);
   } else {
-    bool eno_left = nvat(a, nm) > 2.9;
-    if (d == 0) eno_left = eno_left && (i + 2) < g.mx;
-    recon_weno(a, gv, nm, n0, n1, n2, eno_left, 0, L, R);
+    bool done = false;
+    if constexpr (SCH != 0) {
+      if (weno5_fits(a, id, m, n0, s)) {
+        recon_weno5(a, n0, s, L, R);
+        done = true;
+      }
+    }
+    if (!done) {
+      bool eno_left = nvat(a, nm) > 2.9;
+      if (d == 0) eno_left = eno_left && (i + 2) < g.mx;
+      recon_weno(a, gv, nm, n0, n1, n2, eno_left, 0, L, R);
+    }
   }
   T Fw[5];
-  rusanov(L, R, h, ucon_of(L, h), ucon_of(R, h), Fw);
+  bool roe_done = false;
+  if constexpr (SCH != 0) {
+    if (a.roe) {
+      roe_flux(L, R, h, a.gamma, Fw);
+      roe_done = true;
+    }
+  }
+  if (!roe_done) rusanov(L, R, h, ucon_of(L, h), ucon_of(R, h), Fw);
 #pragma unroll
   for (int q = 0; q < 5; q++) F[q] = Fw[q] * (T(1.0) - sigma) + Fc[q] * sigma;
 }
 
 // instantiate the double version used by the float fallback
-template __device__ void conv_face_les<double>(const StageArgs &, const Sv &, int, int, int, int, double *);
+template __device__ void conv_face_les<double, 0>(const StageArgs &, const Sv &, int, int, int, int, double *);
 
 // ------------------------------ viscous ----------------------------------
 struct Win {  // LDS window of primitives with a one-node halo
@@ -738,7 +790,7 @@ template <class TL> __device__ __forceinline__ int fk_of(int lx, int ly, int lz)
 }
 
 // One workgroup's tile of one grid; b only spreads the norm atomics.
-template <class TL, typename T, bool LES, bool VISC, int MODE>
+template <class TL, typename T, bool LES, bool VISC, int MODE, int SCH = 0>
 __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) {
   constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, WX = TL::WX, WY = TL::WY, NW = TL::NW;
   // The convective and viscous flux of a face are computed in one pass (one
@@ -887,8 +939,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         sv.st = d == 0 ? 1 : d == 1 ? bx : bx * by;
       }
       double F[5];
-      if (LES) conv_face_les<T>(a, sv, d, fk, fj, fi, F);
-      else conv_face_plain<T>(a, sv, d, fk, fj, fi, F);
+      if (LES) conv_face_les<T, SCH>(a, sv, d, fk, fj, fi, F);
+      else conv_face_plain<T, SCH>(a, sv, d, fk, fj, fi, F);
 #pragma unroll
       for (int q = 0; q < 5; q++) Fs[q][f] = F[q];
       if (MERGE) {
@@ -1032,12 +1084,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   a.po[n] = (a.gamma - 1) * (qn[4] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) * rcp64(qn[0]));
 }
 
-template <class TL, typename T, bool LES, bool VISC, int MODE>
+template <class TL, typename T, bool LES, bool VISC, int MODE, int SCH = 0>
 __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWaves))) void stage_kernel(StageArgs a) {
   // XCD-aware tile numbering: blockIdx % 8 selects the XCD, each XCD walks a
   // contiguous range of tiles and keeps their shared halos in its own L2.
   const int b = blockIdx.x;
-  stage_tile<TL, T, LES, VISC, MODE>(a, (b & 7) * a.per_xcd + (b >> 3), b);
+  stage_tile<TL, T, LES, VISC, MODE, SCH>(a, (b & 7) * a.per_xcd + (b >> 3), b);
 }
 
 // Batched stage: the workgroups of one launch cover the tiles of several
@@ -1063,13 +1115,23 @@ void tile_counts(const Grid &g, StageArgs &a) {
   a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
 }
 
-template <class TL, typename T, bool LES, bool VISC>
+template <class TL, typename T, bool LES, bool VISC, int SCH = 0>
 hipError_t launch_t(const StageArgs &a, int mode, int nblocks, hipStream_t s) {
   if (mode == 0)
-    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 0>), dim3(nblocks), dim3(TL::NT), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 0, SCH>), dim3(nblocks), dim3(TL::NT), 0, s, a);
   else
-    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 1>), dim3(nblocks), dim3(TL::NT), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 1, SCH>), dim3(nblocks), dim3(TL::NT), 0, s, a);
   return hipGetLastError();
+}
+
+// the build-only schemes (weno5 / roe, FP64 only) in the default 8x4x3 tile
+hipError_t launch_schemes(const cles_params &p, StageArgs &a, int mode, hipStream_t s) {
+  tile_counts<TileH>(a.g, a);
+  const int nblocks = a.per_xcd * 8;
+  const bool les = p.les != 0, visc = !p.invicid;
+  if (les) return launch_t<TileH, double, true, true, 1>(a, mode, nblocks, s);
+  return visc ? launch_t<TileH, double, false, true, 1>(a, mode, nblocks, s)
+              : launch_t<TileH, double, false, false, 1>(a, mode, nblocks, s);
 }
 
 // every (precision, LES, viscous) variant of one tile shape
@@ -1136,6 +1198,8 @@ void fill_stage_args(cles_ctx *c, int in, int out, int istage, const double coef
   a.twoD = p.twoD;
   a.sig = c->sig;
   a.sgm = c->sgm;
+  a.weno5 = p.weno5;
+  a.roe = p.roe;
   a.a_rhs = coef[0]; a.a_old = coef[1]; a.a_cur = coef[2];
   a.istage = istage;
   a.norms = c->dnorm;
@@ -1180,6 +1244,10 @@ int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const doubl
   StageArgs a;
   fill_stage_args(c, in, out, istage, coef, a);
   hipError_t e;
+  if (c->p.weno5 || c->p.roe) {
+    CLES_HIP(c, launch_schemes(c->p, a, mode, c->stream));
+    return CLES_OK;
+  }
   switch (c->tile) {
     case 1: e = launch_shape<TileB>(c->p, a, mode, c->stream); break;
     case 2: e = launch_shape<TileC>(c->p, a, mode, c->stream); break;

@@ -84,6 +84,9 @@ typedef struct cles_params {
   int i_homo_filter;     /* -i_homo_filter -j_homo_filter -k_homo_filter      */
   int j_homo_filter;     /* (main.c:791-793): Cs from LM/MM averaged over the  */
   int k_homo_filter;     /* homogeneous directions (les.c:934-1116)           */
+  /* Build-only schemes with no reference counterpart (FP64 form only):   */
+  int weno5;             /* WENO5 (Jiang-Shu) where the reference uses WENO3  */
+  int roe;               /* Roe flux (Harten entropy fix) for the Rusanov flux */
 } cles_params;
 
 typedef struct cles_ctx cles_ctx;

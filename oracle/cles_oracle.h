@@ -65,6 +65,8 @@ typedef struct {
   int tistart;             /* time step the run starts from (main.c:890)     */
   int rstart;              /* restart flag (main.c:810)                      */
   int i_homo_filter, j_homo_filter, k_homo_filter; /* -i/j/k_homo_filter (main.c:791-793) */
+  int weno5;               /* build-only: WENO5 (Jiang-Shu) where the reference uses WENO3 */
+  int roe;                 /* build-only: Roe flux in place of the Rusanov flux            */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;
@@ -146,6 +148,11 @@ double orc_eno_qr(double qm, double q0, double qp, int k);
 double orc_muscl_ql(double qm, double q0, double qp);
 double orc_muscl_qr(double qm, double q0, double qp);
 double orc_minmod(double a, double b);
+/* build-only schemes: WENO5 left state from q[0..4] at the face between q[2]
+ * and q[3]; the Roe flux across the face metric h */
+double orc_weno5_ql(const double q[5]);
+void orc_roe(const double ql[5], double pl, const double qr[5], double pr, const double h[3], double gamma,
+             double out[5]);
 /* sigma_cent with the threshold ks (rhs.c:2334-2357) */
 double orc_sigma_cent(double fL, double f0, double fR, double fRR, double ks);
 double orc_fp_2(const double u[4], const double v[4]);

@@ -90,6 +90,99 @@ static void rusanov6(const double *L, const double *R, const double *h, double u
 static inline double ucon_of(const double *s, const double *h) {
   return h[0] * (s[1] / s[0]) + h[1] * (s[2] / s[0]) + h[2] * (s[3] / s[0]);
 }
+/* ---------------------------------------------------------------------
+ * Build-only schemes selectable beside the reference's (no counterpart in
+ * the reference; the options weno5 and roe):
+ *  - WENO5 of Jiang & Shu (J. Comput. Phys. 126, 1996, eqs. 2.10-2.17): the
+ *    left state at face i+1/2 from q[i-2..i+2], candidates
+ *    (2q0 - 7q1 + 11q2)/6, (-q1 + 5q2 + 2q3)/6, (2q2 + 5q3 - q4)/6, linear
+ *    weights 0.1, 0.6, 0.3, smoothness indicators 13/12 (2nd difference)^2 +
+ *    1/4 (1st difference)^2, alpha = d / (1e-6 + beta)^2; the right state
+ *    mirrors it on q[i+3..i-1].  Applied to the point values as the
+ *    reference applies its WENO3 (rhs.c:2359-2461).
+ *  - The Roe flux (Roe, J. Comput. Phys. 43, 1981) in the contravariant form
+ *    of the reference's Rusanov flux: F = (F(L) + F(R))/2 - |h|/2 sum_k
+ *    |lambda_k| alpha_k r_k with Roe averages across the face normal h/|h|,
+ *    gamma of the equation of state, and Harten's entropy fix on the two
+ *    acoustic waves (|lambda| < delta: (lambda^2 + delta^2) / (2 delta),
+ *    delta = 0.1 c).
+ * --------------------------------------------------------------------- */
+double orc_weno5_ql(const double q[5]) {
+  const double eps = 1e-6, d0 = 0.1, d1 = 0.6, d2 = 0.3;
+  const double b0 = 13.0 / 12.0 * (q[0] - 2 * q[1] + q[2]) * (q[0] - 2 * q[1] + q[2]) +
+                    0.25 * (q[0] - 4 * q[1] + 3 * q[2]) * (q[0] - 4 * q[1] + 3 * q[2]);
+  const double b1 = 13.0 / 12.0 * (q[1] - 2 * q[2] + q[3]) * (q[1] - 2 * q[2] + q[3]) +
+                    0.25 * (q[1] - q[3]) * (q[1] - q[3]);
+  const double b2 = 13.0 / 12.0 * (q[2] - 2 * q[3] + q[4]) * (q[2] - 2 * q[3] + q[4]) +
+                    0.25 * (3 * q[2] - 4 * q[3] + q[4]) * (3 * q[2] - 4 * q[3] + q[4]);
+  const double a0 = d0 / ((eps + b0) * (eps + b0)), a1 = d1 / ((eps + b1) * (eps + b1)),
+               a2 = d2 / ((eps + b2) * (eps + b2));
+  const double p0 = (2 * q[0] - 7 * q[1] + 11 * q[2]) / 6.0, p1 = (-q[1] + 5 * q[2] + 2 * q[3]) / 6.0,
+               p2 = (2 * q[2] + 5 * q[3] - q[4]) / 6.0;
+  return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+}
+
+static double harten(double lam, double delta) {
+  const double a = fabs(lam);
+  return a < delta ? (lam * lam + delta * delta) / (2 * delta) : a;
+}
+
+/* physical contravariant flux of s = {rho,rhoU,rhoV,rhoW,rhoE,p} across h */
+static void phys_flux(const double *s, const double *h, double F[5]) {
+  const double U = (h[0] * s[1] + h[1] * s[2] + h[2] * s[3]) / s[0];
+  F[0] = s[0] * U;
+  F[1] = s[1] * U + s[5] * h[0];
+  F[2] = s[2] * U + s[5] * h[1];
+  F[3] = s[3] * U + s[5] * h[2];
+  F[4] = (s[4] + s[5]) * U;
+}
+
+static void roe6(const double *L, const double *R, const double *h, double gamma, orc_q *Fo) {
+  double FL[5], FR[5];
+  phys_flux(L, h, FL);
+  phys_flux(R, h, FR);
+  const double hn = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+  const double n[3] = {h[0] / hn, h[1] / hn, h[2] / hn};
+  const double uL[3] = {L[1] / L[0], L[2] / L[0], L[3] / L[0]}, uR[3] = {R[1] / R[0], R[2] / R[0], R[3] / R[0]};
+  const double HL = (L[4] + L[5]) / L[0], HR = (R[4] + R[5]) / R[0];
+  const double sL = sqrt(L[0]), sR = sqrt(R[0]), ws = sL + sR;
+  double u[3];
+  for (int c = 0; c < 3; c++) u[c] = (sL * uL[c] + sR * uR[c]) / ws;
+  const double H = (sL * HL + sR * HR) / ws, rho = sL * sR;
+  const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const double c2 = (gamma - 1) * (H - 0.5 * q2), c = sqrt(c2);
+  const double Un = n[0] * u[0] + n[1] * u[1] + n[2] * u[2];
+  const double dr = R[0] - L[0], dp = R[5] - L[5];
+  double du[3];
+  for (int k = 0; k < 3; k++) du[k] = uR[k] - uL[k];
+  const double dUn = n[0] * du[0] + n[1] * du[1] + n[2] * du[2];
+  const double a1 = (dp - rho * c * dUn) / (2 * c2), a2 = dr - dp / c2, a5 = (dp + rho * c * dUn) / (2 * c2);
+  const double delta = 0.1 * c;
+  const double l1 = harten(Un - c, delta), l2 = fabs(Un), l5 = harten(Un + c, delta);
+  double dt[3];  /* tangential velocity jump */
+  for (int k = 0; k < 3; k++) dt[k] = du[k] - dUn * n[k];
+  const double udt = u[0] * dt[0] + u[1] * dt[1] + u[2] * dt[2];
+  double D[5];
+  D[0] = l1 * a1 + l2 * a2 + l5 * a5;
+  for (int k = 0; k < 3; k++)
+    D[1 + k] = l1 * a1 * (u[k] - c * n[k]) + l2 * (a2 * u[k] + rho * dt[k]) + l5 * a5 * (u[k] + c * n[k]);
+  D[4] = l1 * a1 * (H - c * Un) + l2 * (a2 * 0.5 * q2 + rho * udt) + l5 * a5 * (H + c * Un);
+  Fo->rho = 0.5 * (FL[0] + FR[0]) - 0.5 * hn * D[0];
+  Fo->rhoU = 0.5 * (FL[1] + FR[1]) - 0.5 * hn * D[1];
+  Fo->rhoV = 0.5 * (FL[2] + FR[2]) - 0.5 * hn * D[2];
+  Fo->rhoW = 0.5 * (FL[3] + FR[3]) - 0.5 * hn * D[3];
+  Fo->rhoE = 0.5 * (FL[4] + FR[4]) - 0.5 * hn * D[4];
+}
+
+void orc_roe(const double ql[5], double pl, const double qr[5], double pr, const double h[3], double gamma,
+             double out[5]) {
+  double L[6] = {ql[0], ql[1], ql[2], ql[3], ql[4], pl};
+  double R[6] = {qr[0], qr[1], qr[2], qr[3], qr[4], pr};
+  orc_q F;
+  roe6(L, R, h, gamma, &F);
+  out[0] = F.rho; out[1] = F.rhoU; out[2] = F.rhoV; out[3] = F.rhoW; out[4] = F.rhoE;
+}
+
 void orc_rusanov(const double ql[5], double pl, const double qr[5], double pr,
                  const double csih[3], double out[5]) {
   double L[6] = {ql[0], ql[1], ql[2], ql[3], ql[4], pl};
@@ -143,6 +236,30 @@ static void recon_weno(const orc_ctx *c, size_t nm, size_t n0, size_t n1, size_t
   }
 }
 
+/* WENO5 on the six nodes nm2, nm, n0, n1, n2, n3 of a face (stride s) */
+static void recon_weno5(const orc_ctx *c, size_t n0, size_t s, double *L, double *R) {
+  double v[6][6];
+  for (int t = 0; t < 6; t++) load6(c, n0 + (size_t)(t - 2) * s, v[t]);  /* t = node - n0 + 2 */
+  for (int q = 0; q < 6; q++) {
+    const double l[5] = {v[0][q], v[1][q], v[2][q], v[3][q], v[4][q]};
+    const double r[5] = {v[5][q], v[4][q], v[3][q], v[2][q], v[1][q]};
+    L[q] = orc_weno5_ql(l);
+    R[q] = orc_weno5_ql(r);
+  }
+}
+/* the WENO5 stencil of the face lies in the grid and holds no solid node */
+static int weno5_fits(const orc_ctx *c, int id, int m, size_t n0, size_t s) {
+  if (!c->p.weno5 || id < 2 || id > m - 4) return 0;
+  const double *nv = c->lnvert;
+  return !(nv[n0 - 2 * s] > BLANKWALL || nv[n0 - s] > BLANKWALL || nv[n0 + 2 * s] > BLANKWALL ||
+           nv[n0 + 3 * s] > BLANKWALL);
+}
+static void riemann_flux(const orc_ctx *c, const double *L, const double *R, const double *h, double uconL,
+                         double uconR, orc_q *F) {
+  if (c->p.roe) roe6(L, R, h, c->p.gamma, F);
+  else rusanov6(L, R, h, uconL, uconR, F);
+}
+
 /* One face of formConvection (rhs.c:2646-2814 xi, 2817-2971 eta, 2973-3121 zeta). */
 static void face_conv(const orc_ctx *c, int d, int k, int j, int i, orc_q *F) {
   const orc_params *P = &c->p;
@@ -162,6 +279,8 @@ static void face_conv(const orc_ctx *c, int d, int k, int j, int i, orc_q *F) {
     load6(c, n0, L); load6(c, n1, R);
   } else if (P->fluxorder == 2 || (d != 0 && j < 30 && k < 7)) {
     recon_muscl(c, nm, n0, n1, n2, L, R);
+  } else if (weno5_fits(c, id, m, n0, s) && !(d == 1 && j == 1 && iso && bc[2] == 1)) {
+    recon_weno5(c, n0, s, L, R);
   } else {
     int eno_left, mask = 0;
     if (d == 0) {
@@ -185,7 +304,7 @@ static void face_conv(const orc_ctx *c, int d, int k, int j, int i, orc_q *F) {
     uconR = uconL;
   else
     uconR = ucon_of(R, h);
-  rusanov6(L, R, h, uconL, uconR, F);
+  riemann_flux(c, L, R, h, uconL, uconR, F);
 }
 
 static void residual(const orc_ctx *c, orc_q *out, int les_forcing) {
@@ -324,11 +443,13 @@ static void face_conv_les(const orc_ctx *c, int d, int k, int j, int i, orc_q *F
     load6(c, n0, L); load6(c, n1, R);
   } else if (P->fluxorder == 2) {
     recon_muscl(c, nm, n0, n1, n2, L, R);
+  } else if (weno5_fits(c, id, m, n0, s)) {
+    recon_weno5(c, n0, s, L, R);
   } else {
     int eno_left = d == 0 ? (nv[nm] > BLANKWALL && (i + 2) < c->mx) : nv[nm] > BLANKWALL;
     recon_weno(c, nm, n0, n1, n2, eno_left, 0, L, R);
   }
-  rusanov6(L, R, h, ucon_of(L, h), ucon_of(R, h), &Fw);
+  riemann_flux(c, L, R, h, ucon_of(L, h), ucon_of(R, h), &Fw);
   F->rho = Fw.rho * (1.0 - sigma) + Fc.rho * sigma;
   F->rhoU = Fw.rhoU * (1.0 - sigma) + Fc.rhoU * sigma;
   F->rhoV = Fw.rhoV * (1.0 - sigma) + Fc.rhoV * sigma;

@@ -33,6 +33,7 @@ class OrcParams(C.Structure):
         ("twoD", C.c_int), ("testfilter_ik", C.c_int),
         ("tistart", C.c_int), ("rstart", C.c_int),
         ("i_homo_filter", C.c_int), ("j_homo_filter", C.c_int), ("k_homo_filter", C.c_int),
+        ("weno5", C.c_int), ("roe", C.c_int),
     ]
 
 
@@ -86,6 +87,9 @@ def lib():
             f = getattr(L, name)
             f.restype = C.c_double
             f.argtypes = [C.c_double] * 3 + [C.c_int]
+        L.orc_weno5_ql.restype = C.c_double
+        L.orc_weno5_ql.argtypes = [dp]
+        L.orc_roe.argtypes = [dp, C.c_double, dp, C.c_double, dp, C.c_double, dp]
         L.orc_sigma_cent.restype = C.c_double
         L.orc_sigma_cent.argtypes = [C.c_double] * 5
         L.orc_minmod.restype = C.c_double
@@ -132,7 +136,7 @@ def make_params(cfg):
                     s_y_min=0.0, s_z_min=0.0, sigma_thrsh=1e-5, ratio=0.0, les=0, invicid=0,
                     fluxorder=3, eno_order=2, central=2, sutherland=0, isothermal=0,
                     calc_sigma=0, abc=0, twoD=0, testfilter_ik=0, tistart=0, rstart=0,
-                    i_homo_filter=0, j_homo_filter=0, k_homo_filter=0)
+                    i_homo_filter=0, j_homo_filter=0, k_homo_filter=0, weno5=0, roe=0)
     d = dict(defaults)
     d.update({k: v for k, v in cfg.items() if k in defaults})
     p.mx, p.my, p.mz = cfg["mx"], cfg["my"], cfg["mz"]

@@ -72,3 +72,5 @@ def test_create_rejects_unsupported_options_before_device_use():
     assert e == -1 and "LES" in msg
     e, msg = _create_err(twoD=4)
     assert e == -1 and "twoD" in msg
+    e, msg = _create_err(weno5=1, precision=1)
+    assert e == -1 and "FP64" in msg
