@@ -27,6 +27,8 @@ LAYOUT_GLOBAL = 0
 LAYOUT_LOCAL = 1
 PREC_FP64 = 0
 PREC_FP32BULK = 1
+BATCH_PERSISTENT = 0  # one persistent kernel per group runs every stage of a batched step
+BATCH_LAUNCHES = 1    # one batched launch per group and stage (default)
 # bodies with more interpolation nodes are refused by cles_ibm_forces_noalloc
 FORCE_NOALLOC_MAX_NODES = 16384
 # return codes of include/cles.h
@@ -70,6 +72,7 @@ EXPORTS = [
     "cles_batch_last_error", "cles_set_tile", "cles_batch_launch", "cles_batch_wait",
     "cles_set_state_qp", "cles_write_fields", "cles_read_fields", "cles_petsc_vec_write", "cles_petsc_vec_read",
     "cles_get_averages", "cles_ibm_forces", "cles_write_vtk", "cles_ibm_forces_noalloc",
+    "cles_batch_set_mode", "cles_batch_set_spin_limit", "cles_batch_set_graph",
 ]
 
 IBM_INFO_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
@@ -139,6 +142,9 @@ def load_library(path=LIB_PATH):
     L.cles_batch_used.argtypes = [vp]
     L.cles_batch_used.restype = C.c_size_t
     L.cles_batch_set_tile.argtypes = [vp, C.c_int]
+    L.cles_batch_set_mode.argtypes = [vp, C.c_int]
+    L.cles_batch_set_spin_limit.argtypes = [vp, C.c_uint]
+    L.cles_batch_set_graph.argtypes = [vp, C.c_int]
     L.cles_batch_advance.argtypes = [vp, C.c_int]
     L.cles_batch_launch.argtypes = [vp, C.c_int]
     L.cles_batch_wait.argtypes = [vp]
@@ -459,8 +465,23 @@ class Batch:
     def used(self):
         return self.L.cles_batch_used(self.h)
 
+    def size(self):
+        return self.L.cles_batch_size(self.h)
+
     def set_tile(self, tile):
         self._check(self.L.cles_batch_set_tile(self.h, int(tile)))
+
+    def set_mode(self, mode):
+        """BATCH_LAUNCHES (default) or BATCH_PERSISTENT"""
+        self._check(self.L.cles_batch_set_mode(self.h, int(mode)))
+
+    def set_graph(self, on):
+        """replay a captured step (True) or launch each step's kernels (False)"""
+        self._check(self.L.cles_batch_set_graph(self.h, int(bool(on))))
+
+    def set_spin_limit(self, polls):
+        """polls of a stage dependency wait of the persistent kernel before it gives up"""
+        self._check(self.L.cles_batch_set_spin_limit(self.h, int(polls)))
 
     def advance(self, nsteps):
         self._check(self.L.cles_batch_advance(self.h, int(nsteps)))

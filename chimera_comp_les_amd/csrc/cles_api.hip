@@ -105,7 +105,7 @@ int dalloc(cles_ctx *c, double **p, size_t elems) {
     return set_error(c, CLES_ENOMEM, "hipMalloc of %zu bytes failed: %s", elems * sizeof(double), hipGetErrorString(e));
   }
   c->bytes += elems * sizeof(double);
-  CLES_HIP(c, hipMemset(*p, 0, elems * sizeof(double)));
+  CLES_HIP(c, hipMemsetAsync(*p, 0, elems * sizeof(double), c->stream));
   return CLES_OK;
 }
 
@@ -280,7 +280,11 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
     }                                   \
   } while (0)
   TRY(hipGetDevice(&c->dev) == hipSuccess ? 0 : set_error(c, CLES_EDEVICE, "no HIP device"));
-  TRY(hipStreamCreate(&c->stream) == hipSuccess ? 0 : set_error(c, CLES_EDEVICE, "hipStreamCreate failed"));
+  // non-blocking: a context's work neither waits for nor holds up the null
+  // stream, other contexts or a batch being captured on another thread
+  TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess
+          ? 0
+          : set_error(c, CLES_EDEVICE, "hipStreamCreate failed"));
   const size_t np = (size_t)g.np;
   TRY(dalloc(c, &c->coor, 3 * np));
   TRY(dalloc(c, &c->csi, 3 * np));
@@ -320,7 +324,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
     void *dn = nullptr;
     TRY(hipMalloc(&dn, 128 * sizeof(unsigned long long)) == hipSuccess ? 0 : set_error(c, CLES_ENOMEM, "norm buffer"));
     c->dnorm = (unsigned long long *)dn;
-    TRY(hipMemset(dn, 0, 128 * sizeof(unsigned long long)) == hipSuccess ? 0 : set_error(c, CLES_EDEVICE, "memset"));
+    TRY(hipMemsetAsync(dn, 0, 128 * sizeof(unsigned long long), c->stream) == hipSuccess ? 0 : set_error(c, CLES_EDEVICE, "memset"));
   }
   TRY(upload(c, coor, c->coor, 3, CLES_LAYOUT_GLOBAL));
   TRY(launch_metrics(c));
@@ -333,6 +337,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
 
 void cles_destroy(cles_ctx *c) {
   if (!c) return;
+  member_leave(c);  // waits for its batch, which then rebuilds without it
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
                     c->fil2, c->qsnap, c->tfil2, c->ivol, c->sabs, c->avg, c->sgm, c->sgm_tmp, c->homo};
@@ -346,6 +351,7 @@ void cles_destroy(cles_ctx *c) {
 
 int cles_set_nvert(cles_ctx *c, const double *nvert) {
   if (!c || !nvert) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   int r = upload(c, nvert, c->nvert, 1, CLES_LAYOUT_GLOBAL);
   if (r) return r;
   int *flag = nullptr;
@@ -363,6 +369,7 @@ int cles_set_nvert(cles_ctx *c, const double *nvert) {
 
 int cles_get_nvert(cles_ctx *c, double *nvert) {
   if (!c || !nvert) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   return download(c, c->nvert, nvert, 1, CLES_LAYOUT_GLOBAL);
 }
 
@@ -370,6 +377,7 @@ int cles_get_nvert(cles_ctx *c, double *nvert) {
 // becomes Q and Qold
 static int set_state(cles_ctx *c, const double *Q, const double *P, int layout) {
   if (!c || !Q) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   // keep the buffer that holds the last CalcUCat input intact
   const int b = c->ucat_src == c->cur ? free_buffer(c, c->cur, -1) : c->cur;
   int r = upload(c, Q, c->Q[b], 5, layout);
@@ -381,8 +389,9 @@ static int set_state(cles_ctx *c, const double *Q, const double *P, int layout) 
   if (c->p.precision == CLES_PREC_FP32BULK) {
     // reference state of the split fluxes: node (1,1,1) of this state
     const long long n = nid(c->g, 1, 1, 1);
-    CLES_HIP(c, hipMemcpy(&c->ref_rho, c->Q[b] + n, sizeof(double), hipMemcpyDeviceToHost));
-    CLES_HIP(c, hipMemcpy(&c->ref_p, c->P[b] + n, sizeof(double), hipMemcpyDeviceToHost));
+    CLES_HIP(c, hipMemcpyAsync(&c->ref_rho, c->Q[b] + n, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CLES_HIP(c, hipMemcpyAsync(&c->ref_p, c->P[b] + n, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CLES_HIP(c, hipStreamSynchronize(c->stream));
     if (!(c->ref_rho > 0.0)) return set_error(c, CLES_EINVAL, "non-positive density at node (1,1,1)");
   }
   c->have_state = 1;
@@ -398,6 +407,7 @@ int cles_set_state_qp(cles_ctx *c, const double *Q, const double *P, int layout)
 
 int cles_get_state(cles_ctx *c, double *Q, double *P, int layout) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   if (Q) {
     int r = download(c, c->Q[c->cur], Q, 5, layout);
@@ -412,6 +422,7 @@ int cles_get_state(cles_ctx *c, double *Q, double *P, int layout) {
 
 int cles_get_metrics(cles_ctx *c, double *csi, double *eta, double *zet, double *aj) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   int r = 0;
   if (csi && (r = download(c, c->csi, csi, 3, CLES_LAYOUT_GLOBAL))) return r;
   if (eta && (r = download(c, c->eta, eta, 3, CLES_LAYOUT_GLOBAL))) return r;
@@ -422,6 +433,7 @@ int cles_get_metrics(cles_ctx *c, double *csi, double *eta, double *zet, double 
 
 int cles_get_averages(cles_ctx *c, double *q_sum, double *p_sum, double *u2_sum, double *ucross_sum) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   if (!c->avg) return set_error(c, CLES_ESTATE, "averaging is off (cles_params.averaging = 0)");
   const long long np = c->g.np;
   int r;
@@ -434,6 +446,7 @@ int cles_get_averages(cles_ctx *c, double *q_sum, double *p_sum, double *u2_sum,
 
 int cles_get_les(cles_ctx *c, double *cs, double *nu_t, double *sigma) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   const long long N = nodes(c);
   int r = 0;
   if (cs) {
@@ -466,6 +479,7 @@ int cles_get_les(cles_ctx *c, double *cs, double *nu_t, double *sigma) {
 
 int cles_eq_of_state(cles_ctx *c) {
   if (!c || !c->have_state) return CLES_ESTATE;
+  if (int g_ = member_guard(c, true)) return g_;
   int r = launch_eos_all(c, c->cur);
   if (r) return r;
   CLES_HIP(c, hipStreamSynchronize(c->stream));
@@ -474,6 +488,7 @@ int cles_eq_of_state(cles_ctx *c) {
 
 int cles_form_rhs(cles_ctx *c, int istage, double *rhs) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   const double coef[3] = {0, 0, 0};
   CLES_HIP(c, hipMemsetAsync(c->rhs, 0, 5 * c->g.np * sizeof(double), c->stream));
@@ -528,6 +543,7 @@ static int runge_kutta_async(cles_ctx *c) {
 
 int cles_runge_kutta(cles_ctx *c) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   int r = runge_kutta_async(c);
   if (r) return r;
@@ -537,6 +553,7 @@ int cles_runge_kutta(cles_ctx *c) {
 
 int cles_les_update(cles_ctx *c) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   if (!c->p.les) return set_error(c, CLES_EINVAL, "LES is off (-les 0)");
   int r = launch_les(c);
@@ -560,6 +577,7 @@ static int flow_step_async(cles_ctx *c) {
 
 int cles_flow_step(cles_ctx *c) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   int r = flow_step_async(c);
   if (r) return r;
@@ -569,6 +587,7 @@ int cles_flow_step(cles_ctx *c) {
 
 int cles_advance(cles_ctx *c, int nsteps) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   for (int s = 0; s < nsteps; s++) {
     int r = flow_step_async(c);
@@ -580,6 +599,7 @@ int cles_advance(cles_ctx *c, int nsteps) {
 
 int cles_norms(cles_ctx *c, double out[2]) {
   if (!c || !out) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   unsigned long long h[128];
   CLES_HIP(c, hipMemcpyAsync(h, c->dnorm, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   CLES_HIP(c, hipStreamSynchronize(c->stream));
@@ -596,12 +616,14 @@ int cles_norms(cles_ctx *c, double out[2]) {
 int cles_get_time(cles_ctx *c) { return c ? c->ti : CLES_EINVAL; }
 int cles_set_time(cles_ctx *c, int ti) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   c->ti = ti;
   return CLES_OK;
 }
 
 int cles_time_stage_kernel(cles_ctx *c, int nrep, double *ms) {
   if (!c || !ms || nrep < 1) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   const int out = free_buffer(c, c->cur, c->ucat_src);
   double coef[3], alfa;
@@ -628,6 +650,7 @@ int cles_time_stage_kernel(cles_ctx *c, int nrep, double *ms) {
 
 int cles_time_les(cles_ctx *c, int nrep, double *ms) {
   if (!c || !ms || nrep < 1) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   if (!c->p.les) return set_error(c, CLES_EINVAL, "LES is off");
   hipEvent_t e0, e1;

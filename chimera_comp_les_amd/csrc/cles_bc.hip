@@ -13,6 +13,7 @@
 #include <cstring>
 
 #include "cles_internal.hpp"
+#include "cles_seam.hpp"
 
 namespace cles {
 
@@ -35,48 +36,7 @@ struct BndArgs {
   long long nb;  // number of boundary nodes
 };
 
-// linear boundary index -> (k, j, i), every boundary node exactly once
-__device__ __forceinline__ void bnd_decode(const Grid &g, long long b, int &k, int &j, int &i) {
-  const long long mx = g.mx, my = g.my, mz = g.mz;
-  const long long pk = mx * my;              // k = 0 and k = mz-1 planes
-  const long long pj = mx * (mz - 2);        // j = 0 and j = my-1, 1<=k<=mz-2
-  if (b < 2 * pk) {
-    k = b < pk ? 0 : (int)(mz - 1);
-    long long r = b % pk;
-    j = (int)(r / mx);
-    i = (int)(r % mx);
-    return;
-  }
-  b -= 2 * pk;
-  if (b < 2 * pj) {
-    j = b < pj ? 0 : (int)(my - 1);
-    long long r = b % pj;
-    k = 1 + (int)(r / mx);
-    i = (int)(r % mx);
-    return;
-  }
-  b -= 2 * pj;
-  // i = 0 and i = mx-1 with interior j, k: (my-2)(mz-2) nodes each.  The two
-  // nodes of one (j, k) go to neighbouring threads:
-  // each shares its 128 B line with the other's seam source (i = 1, mx-2)
-  i = (b & 1) ? (int)(mx - 1) : 0;
-  const long long r = b >> 1;
-  k = 1 + (int)(r / (my - 2));
-  j = 1 + (int)(r % (my - 2));
-}
-
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
-
-// Source node of the periodic seam copy that finally owns (k,j,i), or -1.
-__device__ __forceinline__ long long seam_src(const Grid &g, int k, int j, int i) {
-  if (g.pz && (k == 0 || k == g.mz - 1)) return nid(g, k == 0 ? g.mz - 2 : 1, j, i);
-  if (g.py) {
-    if (j == 0 && k >= 1) return nid(g, k, g.my - 2, i);
-    if (j == g.my - 1 && k >= 1 && k <= g.mz - 2) return nid(g, k, 1, i);
-  }
-  if (g.px && (i == 0 || i == g.mx - 1) && k >= 1 && j >= 1) return nid(g, k, j, i == 0 ? g.mx - 2 : 1);
-  return -1;
-}
 
 // stage update with a zero right-hand side on every boundary node, plus the
 // boundary share of |Q_in|_inf
@@ -411,21 +371,7 @@ __global__ __launch_bounds__(256) void bc_k_kernel(BndArgs a, int ls, int le) {
 // periodic seams from the stage input and the equation of state on every
 // boundary node
 __device__ __forceinline__ void seam_eos_node(const BndArgs &a, long long b) {
-  const Grid &g = a.g;
-  const long long np = g.np;
-  if (b >= a.nb) return;
-  int k, j, i;
-  bnd_decode(g, b, k, j, i);
-  const long long n = nid(g, k, j, i);
-  double qn[5];
-  const long long s = a.periodic ? seam_src(g, k, j, i) : -1;
-#pragma unroll
-  for (int q = 0; q < 5; q++) qn[q] = s >= 0 ? a.q[s + q * np] : a.qo[n + q * np];
-  if (s >= 0) {
-#pragma unroll
-    for (int q = 0; q < 5; q++) a.qo[n + q * np] = qn[q];
-  }
-  a.po[n] = (a.gamma - 1) * (qn[4] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) / qn[0]);
+  seam_copy_eos(a.g, a.q, a.qo, a.po, a.gamma, a.periodic, b, a.nb);
 }
 
 __global__ __launch_bounds__(256) void seam_eos_kernel(BndArgs a) {

@@ -128,6 +128,7 @@ using namespace cles;
 
 extern "C" int cles_integrals(cles_ctx *c, double out[6]) {
   if (!c || !out) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   double *buf = nullptr;
   CLES_HIP(c, hipMalloc((void **)&buf, (6 * kRedBlocks + 6) * sizeof(double)));

@@ -320,6 +320,7 @@ int forces_run(cles_ctx *c, const double center[3], double out[CLES_FORCE_N], do
 
 extern "C" int cles_ibm_forces(cles_ctx *c, const double center[3], double out[CLES_FORCE_N]) {
   if (!c || !center || !out) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   cles_ibm_dev *b = c->ibm;
   if (!b) return set_error(c, CLES_ESTATE, "no immersed body (cles_ibm_search)");
@@ -378,6 +379,7 @@ static int scratch_address(cles_ctx *c, double **ptr) {
 
 extern "C" int cles_ibm_forces_noalloc(cles_ctx *c, const double center[3], double out[CLES_FORCE_N]) {
   if (!c || !center || !out) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   if (!c->have_state) return cles::set_error(c, CLES_ESTATE, "no state uploaded");
   cles_ibm_dev *b = c->ibm;
   if (!b) return cles::set_error(c, CLES_ESTATE, "no immersed body (cles_ibm_search)");

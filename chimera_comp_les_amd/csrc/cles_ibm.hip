@@ -854,6 +854,7 @@ int cles_ibm_search(cles_ctx *c, int n_v, const double *x_bp, const double *y_bp
                     const int *nv1, const int *nv2, const int *nv3, const double *nvert_o, int *n_ib) {
   if (!c || n_v < 3 || n_elmt < 1 || !x_bp || !y_bp || !z_bp || !nv1 || !nv2 || !nv3)
     return set_error(c, CLES_EINVAL, "cles_ibm_search: empty surface or null pointer");
+  if (int g_ = member_guard(c, true)) return g_;
   for (int e = 0; e < n_elmt; e++)
     if (nv1[e] < 0 || nv1[e] >= n_v || nv2[e] < 0 || nv2[e] >= n_v || nv3[e] < 0 || nv3[e] >= n_v)
       return set_error(c, CLES_EINVAL, "cles_ibm_search: triangle %d names a vertex outside 0..%d", e, n_v - 1);
@@ -1063,6 +1064,7 @@ int cles_ibm_search(cles_ctx *c, int n_v, const double *x_bp, const double *y_bp
 
 int cles_ibm_set_velocity(cles_ctx *c, const double *u, const double *uold) {
   if (!c || !u || !uold) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   cles_ibm_dev *b = c->ibm;
   if (!b) return set_error(c, CLES_ESTATE, "no immersed surface (call cles_ibm_search first)");
   const int nv = b->nv;
@@ -1080,6 +1082,7 @@ int cles_ibm_set_velocity(cles_ctx *c, const double *u, const double *uold) {
 
 int cles_ibm_get_list(cles_ctx *c, cles_ibm_info *out, int n) {
   if (!c || (!out && n > 0)) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   cles_ibm_dev *b = c->ibm;
   const int nib = b ? b->n_ib : 0;
   if (n != nib) return set_error(c, CLES_EINVAL, "cles_ibm_get_list: list holds %d nodes, %d requested", nib, n);

@@ -240,8 +240,11 @@ struct cles_ibm_dev {
   double *frec, *fout;   // surface-force records (n_ib of them) and outputs, made at the first call
 };
 
+struct cles_batch;
+
 struct cles_ctx {
   cles_params p;
+  cles_batch *batch;  // the batch the context is a member of, or null
   cles::Grid g;
   int dev;
   hipStream_t stream;
@@ -297,17 +300,60 @@ bool supported_bc(int side, int b);
 int launch_les(cles_ctx *c);
 // -calc_sigma: the shock sensor of the state in buffer buf into c->sgm
 int launch_sigma(cles_ctx *c, int buf);
+// the same on another stream (batched steps); the LES model on stream s with
+// its stale state read through the device cell qs_cell
+int launch_sigma_on(cles_ctx *c, int buf, hipStream_t s);
+int launch_les_on(cles_ctx *c, hipStream_t s, const double *const *qs_cell);
 int launch_averaging(cles_ctx *c);
 // node coordinates, interleaved x y z in the natural order (global layout)
 int get_coor(cles_ctx *c, double *xyz);
 // batched stages of several periodic grids (serving)
 size_t stage_args_bytes();
 int stage_batch_args(cles_ctx *c, int in, int out, int istage, const double coef[3], int tile, void *dst);
-hipError_t launch_stage_batch(int tile, int fp32, int visc, const void *tab, const int *tstart, int nreq, int nblocks,
+hipError_t launch_stage_batch(int tile, int fp32, int visc, int les, const void *tab, const int *tstart, int nreq, int nblocks,
                               hipStream_t s);
 size_t seam_args_bytes();
 int seam_batch_args(cles_ctx *c, int in, int out, void *dst);
 hipError_t launch_seam_batch(const void *tab, const int *bstart, int nreq, int nblocks, hipStream_t s);
+
+// Persistent batched stages: one launch per (precision, model) group covers
+// every stage of every member of the group.  The items (a run of tiles, or
+// kSeamItem boundary nodes) sit in eight queues, one per XCD; an item of
+// stage s of a member waits until the member's completion counter shows
+// every item of its stage s-1 done.
+constexpr int kSeamItem = 512;
+// the queue heads and completion counters sit 256 B apart: polls and
+// increments of different counters go to different memory channels
+constexpr int kCtrStride = 64;
+struct PersistPhase {  // the share of one (stage, member) in one queue
+  int member, stage;
+  int t0, nt, grain;  // tiles [t0, t0 + nt), grain tiles per item
+  int b0, nb;         // seam items [b0, b0 + nb)
+  unsigned need;      // done[member] the items of the phase wait for
+  int istart;         // first item of the phase in the queue (tiles, then seams)
+};
+struct SeamArgs {
+  Grid g;
+  const double *q;  // stage input
+  double *qo, *po;  // stage output
+  double gamma;
+  long long nb;     // boundary nodes
+};
+struct PersistArgs {
+  const void *stage;          // stage arguments [stage][member slot]
+  const SeamArgs *seam;       // [stage][member slot]
+  const PersistPhase *phase;  // [queue][pstride], the first nph of each queue used
+  const int *nitems;          // [queue]
+  int nph, pstride, cap;
+  unsigned *head;             // [queue * kCtrStride] next item (zero at launch)
+  unsigned *done;             // [member slot * kCtrStride] items done (zero at launch)
+  unsigned *abort;            // set when a wait gave up (the heads are then pushed past the end)
+  unsigned spin_limit;        // polls of a dependency wait before it gives up
+};
+hipError_t launch_stage_persist(int tile, int fp32, int visc, int les, const PersistArgs &a, hipStream_t s);
+// workgroups of launch_stage_persist that fit on the device at once
+int persist_blocks(int tile, int fp32, int visc, int les);
+int stage_tiles(cles_ctx *c, int tile);
 int rk_coef(const cles_params &p, int istage, double coef[3]);
 size_t ctx_footprint(const cles_params *p);
 // immersed body
@@ -316,6 +362,11 @@ int launch_snapshot(cles_ctx *c, int buf);
 int launch_boundary_post(cles_ctx *c, int buf);
 void ibm_free(cles_ctx *c);
 int set_error(cles_ctx *c, int code, const char *fmt, ...);
+// A context in a batch: CLES_ESTATE while the batch runs; a call that
+// changes the context (modifies) marks the batch for a rebuild.
+int member_guard(cles_ctx *c, bool modifies);
+// leave the batch the context is a member of (waits for the batch)
+void member_leave(cles_ctx *c);
 }  // namespace cles
 
 #define CLES_HIP(c, call)                                                                      \

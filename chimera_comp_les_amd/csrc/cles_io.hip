@@ -106,6 +106,7 @@ int cles_petsc_vec_read(const char *path, double *x, long long n) {
 
 int cles_write_fields(cles_ctx *c, const char *dir, int block) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
   std::vector<double> q((size_t)N * 5), p((size_t)N), nv((size_t)N);
@@ -149,6 +150,7 @@ int cles_write_fields(cles_ctx *c, const char *dir, int block) {
 
 int cles_read_fields(cles_ctx *c, const char *dir, int ti, int block) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, true)) return g_;
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
   std::vector<double> q((size_t)N * 5), p((size_t)N), nv((size_t)N);
   char path[1024];
@@ -193,6 +195,7 @@ struct VtkArray {
 
 extern "C" int cles_write_vtk(cles_ctx *c, const char *dir, int block) {
   if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   const int MX = c->g.mx, MY = c->g.my, MZ = c->g.mz;
   const long long N = (long long)MX * MY * MZ;

@@ -38,6 +38,9 @@ struct LesArgs {
   Grid g;
   const double *q;   // current state (lQ)
   const double *qs;  // state of the last CalcUCat, or null (lUcat still zero)
+  // batched steps: the address of qs read from a device cell that the
+  // captured step updates (null: use qs)
+  const double *const *qs_cell;
   const double *csi, *eta, *zet, *aj, *nv;
   int has_ib;
   T *grad;  // 7 planes: S11 S12 S13 S22 S23 S33 (no trace removal) and |S|
@@ -72,8 +75,9 @@ template <typename T> __device__ __forceinline__ void vel(const double *q, long 
 }
 // velocity of the last CalcUCat (zero before the first)
 template <typename T> __device__ __forceinline__ void u_stale(const LesArgs<T> &a, long long n, T u[3]) {
-  if (a.qs) {
-    vel(a.qs, n, a.g.np, u);
+  const double *qs = a.qs_cell ? *a.qs_cell : a.qs;
+  if (qs) {
+    vel(qs, n, a.g.np, u);
     if (a.twoD == 1) u[0] = T(0);
   } else {
     u[0] = u[1] = u[2] = T(0);
@@ -619,12 +623,13 @@ __global__ __launch_bounds__(256) void fill_kernel(double *x, long long n, doubl
 }
 
 template <typename T>
-int launch_les_t(cles_ctx *c) {
+int launch_les_t(cles_ctx *c, hipStream_t stream, const double *const *qs_cell) {
   const cles_params &p = c->p;
   LesArgs<T> a;
   a.g = c->g;
   a.q = c->Q[c->cur];
   a.qs = c->ucat_src >= 0 ? c->Q[c->ucat_src] : nullptr;
+  a.qs_cell = qs_cell;
   a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert;
   a.has_ib = c->has_ib;
   a.grad = (T *)c->grad; a.lm = (T *)c->lm; a.mm = (T *)c->mm; a.sabs = (T *)c->sabs;
@@ -650,36 +655,36 @@ int launch_les_t(cles_ctx *c) {
   if (early || p.les == 1) {
     // les.c:139-151: Cs = 0 or 0.03 everywhere; lUcat keeps the velocity of
     // the last CalcUCat
-    hipLaunchKernelGGL(fill_kernel, dim3(gN), dim3(bs), 0, c->stream, c->cs, N, early ? 0.0 : 0.03);
+    hipLaunchKernelGGL(fill_kernel, dim3(gN), dim3(bs), 0, stream, c->cs, N, early ? 0.0 : 0.03);
     CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(les_nut_kernel<T>, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
+    hipLaunchKernelGGL(les_nut_kernel<T>, dim3(gX), dim3(bs), 0, stream, a, per_xcd);
     CLES_HIP(c, hipGetLastError());
     return CLES_OK;
   }
-  hipLaunchKernelGGL(les_grad_kernel<T>, dim3(gX), dim3(bs), 0, c->stream, a, per_xcd);
+  hipLaunchKernelGGL(les_grad_kernel<T>, dim3(gX), dim3(bs), 0, stream, a, per_xcd);
   CLES_HIP(c, hipGetLastError());
   {
     const int ntx = (c->g.mx - 2 + FTX - 1) / FTX, nty = (c->g.my - 2 + FTY - 1) / FTY,
               ntz = (c->g.mz - 2 + FTZ - 1) / FTZ;
     const int ntiles = ntx * nty * ntz, pxf = (ntiles + 7) / 8;
-    hipLaunchKernelGGL(les_filter_kernel<T>, dim3(8 * pxf), dim3(256), 0, c->stream, a, ntx, nty, ntiles, pxf);
+    hipLaunchKernelGGL(les_filter_kernel<T>, dim3(8 * pxf), dim3(256), 0, stream, a, ntx, nty, ntiles, pxf);
   }
   CLES_HIP(c, hipGetLastError());
   if (a.ih) {  // the averaged branches (-j or -k alone average nothing)
     const long long L = (long long)c->g.my * c->g.mz;
-    hipLaunchKernelGGL(homo_line_kernel<T>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(homo_line_kernel<T>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, stream, a);
     CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(homo_plane_kernel<T>, dim3((unsigned)((c->g.my + 255) / 256)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(homo_plane_kernel<T>, dim3((unsigned)((c->g.my + 255) / 256)), dim3(256), 0, stream, a);
     CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(homo_global_kernel<T>, dim3(1), dim3(64), 0, c->stream, a);
+    hipLaunchKernelGGL(homo_global_kernel<T>, dim3(1), dim3(64), 0, stream, a);
     CLES_HIP(c, hipGetLastError());
   }
   {
     const int ntx = (c->g.mx + MTX - 1) / MTX, nty = (c->g.my + MTY - 1) / MTY, ntz = (c->g.mz + MTZ - 1) / MTZ;
     const int ntiles = ntx * nty * ntz, pxm = (ntiles + 7) / 8;
-    hipLaunchKernelGGL(les_cs_kernel<T>, dim3(8 * pxm), dim3(256), 0, c->stream, a, ntx, nty, ntiles, pxm);
+    hipLaunchKernelGGL(les_cs_kernel<T>, dim3(8 * pxm), dim3(256), 0, stream, a, ntx, nty, ntiles, pxm);
     CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(les_smooth_kernel<T>, dim3(8 * pxm), dim3(256), 0, c->stream, a, ntx, nty, ntiles, pxm);
+    hipLaunchKernelGGL(les_smooth_kernel<T>, dim3(8 * pxm), dim3(256), 0, stream, a, ntx, nty, ntiles, pxm);
     CLES_HIP(c, hipGetLastError());
   }
   double *t = c->cs;
@@ -692,7 +697,15 @@ int launch_les_t(cles_ctx *c) {
 }  // namespace
 
 int launch_les(cles_ctx *c) {
-  return c->p.precision == CLES_PREC_FP32BULK ? launch_les_t<float>(c) : launch_les_t<double>(c);
+  return c->p.precision == CLES_PREC_FP32BULK ? launch_les_t<float>(c, c->stream, nullptr)
+                                              : launch_les_t<double>(c, c->stream, nullptr);
+}
+
+// The LES model on stream s with the stale state read through the device
+// cell qs_cell (batched, graph-captured steps)
+int launch_les_on(cles_ctx *c, hipStream_t s, const double *const *qs_cell) {
+  return c->p.precision == CLES_PREC_FP32BULK ? launch_les_t<float>(c, s, qs_cell)
+                                              : launch_les_t<double>(c, s, qs_cell);
 }
 
 }  // namespace cles

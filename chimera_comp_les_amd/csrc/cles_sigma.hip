@@ -128,8 +128,9 @@ __global__ __launch_bounds__(256) void sigma_curv_kernel(SigArgs a, const double
 
 }  // namespace
 
-// The sensor of the state in buffer buf into c->sgm (calc_sigma only).
-int launch_sigma(cles_ctx *c, int buf) {
+// The sensor of the state in buffer buf into c->sgm (calc_sigma only), on
+// stream s.
+int launch_sigma_on(cles_ctx *c, int buf, hipStream_t s) {
   const cles_params &p = c->p;
   SigArgs a;
   a.g = c->g;
@@ -143,13 +144,15 @@ int launch_sigma(cles_ctx *c, int buf) {
   a.zper = p.bctype[4] == 7;
   const long long N = (long long)c->g.mx * c->g.my * c->g.mz;
   const dim3 grid((unsigned)((N + 255) / 256)), blk(256);
-  hipLaunchKernelGGL(sigma_raw_kernel, grid, blk, 0, c->stream, a);
+  hipLaunchKernelGGL(sigma_raw_kernel, grid, blk, 0, s, a);
   CLES_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(sigma_smooth_kernel, grid, blk, 0, c->stream, a);
+  hipLaunchKernelGGL(sigma_smooth_kernel, grid, blk, 0, s, a);
   CLES_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(sigma_curv_kernel, grid, blk, 0, c->stream, a, (const double *)c->sgm_tmp, c->sgm);
+  hipLaunchKernelGGL(sigma_curv_kernel, grid, blk, 0, s, a, (const double *)c->sgm_tmp, c->sgm);
   CLES_HIP(c, hipGetLastError());
   return CLES_OK;
 }
+
+int launch_sigma(cles_ctx *c, int buf) { return launch_sigma_on(c, buf, c->stream); }
 
 }  // namespace cles

@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "cles_internal.hpp"
+#include "cles_seam.hpp"
 
 namespace cles {
 
@@ -1094,7 +1095,7 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWa
 
 // Batched stage: the workgroups of one launch cover the tiles of several
 // independent grids; tstart[r] is the first workgroup of grid r (prefix sum
-// of the tile counts, tstart[nreq] = total).
+// of the workgroup counts, 8 per_xcd each, tstart[nreq] = total).
 template <class TL, typename T, bool LES, bool VISC, int MODE>
 __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWaves))) void stage_batch_kernel(const StageArgs *__restrict__ tab, const int *__restrict__ tstart,
                                                          int nreq) {
@@ -1104,7 +1105,94 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWa
     const int mid = (lo + hi) >> 1;
     if (tstart[mid] <= b) lo = mid; else hi = mid;
   }
-  stage_tile<TL, T, LES, VISC, MODE>(tab[lo], b - tstart[lo], b);
+  // the grid's own XCD-aware numbering (every grid starts on a multiple of 8)
+  const int lb = b - tstart[lo];
+  stage_tile<TL, T, LES, VISC, MODE>(tab[lo], (lb & 7) * tab[lo].per_xcd + (lb >> 3), b);
+}
+
+// Persistent batched stages (serving): the workgroups pull items from eight
+// queues, their own XCD's first (blockIdx % 8), then the others.  Items are
+// queued stage by stage, so an item waits only for items handed out before
+// it; every one of those is held by a running workgroup, whose own waits
+// again point only backwards, so the waits always end.  A wait is still
+// bounded: after spin_limit polls it sets the abort flag and pushes every
+// queue head past its end, the other waits see the flag within 32 polls,
+// every workgroup stops taking items, the kernel exits and the host reports
+// the failure.
+template <class TL, typename T, bool LES, bool VISC>
+__global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWaves))) void stage_persist_kernel(PersistArgs a) {
+  __shared__ int s_item;
+  const StageArgs *tab = (const StageArgs *)a.stage;
+  const int home = blockIdx.x & 7;
+  for (int qq = 0; qq < 8; qq++) {
+    const int x = (home + qq) & 7;
+    const int n = a.nitems[x];
+    const PersistPhase *ph0 = a.phase + (size_t)x * a.pstride;
+    for (;;) {
+      __syncthreads();  // the previous item's LDS is free
+      if (threadIdx.x == 0) {
+        const unsigned h = __hip_atomic_fetch_add(&a.head[x * kCtrStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_item = h < (unsigned)n ? (int)h : -1;
+      }
+      __syncthreads();
+      int it = s_item;
+      if (it < 0) break;
+      int lo = 0, hi = a.nph;  // the last phase starting at or before it
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ph0[mid].istart <= it) lo = mid; else hi = mid;
+      }
+      const PersistPhase ph = ph0[lo];
+      if (ph.need) {
+        if (threadIdx.x == 0) {
+          unsigned spins = 0;
+          while (__hip_atomic_load(&a.done[ph.member * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ph.need) {
+            ++spins;
+            if (spins > a.spin_limit ||
+                ((spins & 31) == 0 && __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+              // give up: flag the failure and push every queue past its end
+              __hip_atomic_store(a.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              for (int y = 0; y < 8; y++)
+                __hip_atomic_fetch_add(&a.head[y * kCtrStride], 1u << 30, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              s_item = -1;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+          }
+          // the producers' stores (released before their counter increments)
+          // become visible to this workgroup
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+        it = s_item;
+        if (it < 0) break;
+      }
+      const int local = it - ph.istart;
+      const int nti = (ph.nt + ph.grain - 1) / ph.grain;
+      if (local < nti) {
+        const StageArgs &sa = tab[(size_t)ph.stage * a.cap + ph.member];
+        const int tb = ph.t0 + local * ph.grain;
+        const int te = tb + ph.grain < ph.t0 + ph.nt ? tb + ph.grain : ph.t0 + ph.nt;
+        for (int t = tb; t < te; t++) {
+          if (t != tb) __syncthreads();
+          stage_tile<TL, T, LES, VISC, 1>(sa, t, it);
+        }
+      } else {
+        const SeamArgs &sm = a.seam[(size_t)ph.stage * a.cap + ph.member];
+        const long long b0 = (long long)(ph.b0 + local - nti) * kSeamItem;
+        for (int o = threadIdx.x; o < kSeamItem; o += TL::NT)
+          seam_copy_eos(sm.g, sm.q, sm.qo, sm.po, sm.gamma, 1, b0 + o, sm.nb);
+      }
+      // every wave's stores have reached the L2 before the workgroup
+      // releases them and counts the item
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(&a.done[ph.member * kCtrStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
 }
 
 template <class TL>
@@ -1148,19 +1236,63 @@ hipError_t launch_shape(const cles_params &p, StageArgs &a, int mode, hipStream_
   return visc ? launch_t<TL, double, false, true>(a, mode, nblocks, s) : launch_t<TL, double, false, false>(a, mode, nblocks, s);
 }
 
-template <class TL, typename T, bool VISC>
+template <class TL, typename T, bool LES, bool VISC>
 hipError_t launch_batch_t(const StageArgs *tab, const int *tstart, int nreq, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((stage_batch_kernel<TL, T, false, VISC, 1>), dim3(nblocks), dim3(TL::NT), 0, s, tab, tstart, nreq);
+  hipLaunchKernelGGL((stage_batch_kernel<TL, T, LES, VISC, 1>), dim3(nblocks), dim3(TL::NT), 0, s, tab, tstart, nreq);
   return hipGetLastError();
 }
 
 template <class TL>
-hipError_t launch_batch_shape(int fp32, int visc, const StageArgs *tab, const int *tstart, int nreq, int nblocks,
-                              hipStream_t s) {
-  if (fp32) return visc ? launch_batch_t<TL, float, true>(tab, tstart, nreq, nblocks, s)
-                        : launch_batch_t<TL, float, false>(tab, tstart, nreq, nblocks, s);
-  return visc ? launch_batch_t<TL, double, true>(tab, tstart, nreq, nblocks, s)
-              : launch_batch_t<TL, double, false>(tab, tstart, nreq, nblocks, s);
+hipError_t launch_batch_shape(int fp32, int visc, int les, const StageArgs *tab, const int *tstart, int nreq,
+                              int nblocks, hipStream_t s) {
+  if (les) {  // the LES forms use the 8x4x3 tile
+    if (fp32) return visc ? launch_batch_t<TileH, float, true, true>(tab, tstart, nreq, nblocks, s)
+                          : launch_batch_t<TileH, float, true, false>(tab, tstart, nreq, nblocks, s);
+    return visc ? launch_batch_t<TileH, double, true, true>(tab, tstart, nreq, nblocks, s)
+                : launch_batch_t<TileH, double, true, false>(tab, tstart, nreq, nblocks, s);
+  }
+  if (fp32) return visc ? launch_batch_t<TL, float, false, true>(tab, tstart, nreq, nblocks, s)
+                        : launch_batch_t<TL, float, false, false>(tab, tstart, nreq, nblocks, s);
+  return visc ? launch_batch_t<TL, double, false, true>(tab, tstart, nreq, nblocks, s)
+              : launch_batch_t<TL, double, false, false>(tab, tstart, nreq, nblocks, s);
+}
+
+template <class TL, typename T, bool LES, bool VISC>
+int persist_blocks_t() {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stage_persist_kernel<TL, T, LES, VISC>, TL::NT, 0) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  return per_cu * cus;
+}
+
+template <class TL, typename T, bool LES, bool VISC>
+hipError_t persist_t(const PersistArgs &a, hipStream_t s) {
+  const int nb = persist_blocks_t<TL, T, LES, VISC>();
+  if (nb < 8) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((stage_persist_kernel<TL, T, LES, VISC>), dim3(nb & ~7), dim3(TL::NT), 0, s, a);
+  return hipGetLastError();
+}
+
+// (precision, model) -> instantiation; the LES forms use the 8x4x3 tile
+template <class TL>
+hipError_t persist_shape(int fp32, int visc, int les, const PersistArgs &a, hipStream_t s) {
+  if (les) {
+    if (fp32) return visc ? persist_t<TileH, float, true, true>(a, s) : persist_t<TileH, float, true, false>(a, s);
+    return visc ? persist_t<TileH, double, true, true>(a, s) : persist_t<TileH, double, true, false>(a, s);
+  }
+  if (fp32) return visc ? persist_t<TL, float, false, true>(a, s) : persist_t<TL, float, false, false>(a, s);
+  return visc ? persist_t<TL, double, false, true>(a, s) : persist_t<TL, double, false, false>(a, s);
+}
+
+template <class TL>
+int persist_blocks_shape(int fp32, int visc, int les) {
+  if (les) {
+    if (fp32) return visc ? persist_blocks_t<TileH, float, true, true>() : persist_blocks_t<TileH, float, true, false>();
+    return visc ? persist_blocks_t<TileH, double, true, true>() : persist_blocks_t<TileH, double, true, false>();
+  }
+  if (fp32) return visc ? persist_blocks_t<TL, float, false, true>() : persist_blocks_t<TL, float, false, false>();
+  return visc ? persist_blocks_t<TL, double, false, true>() : persist_blocks_t<TL, double, false, false>();
 }
 
 }  // namespace
@@ -1222,21 +1354,55 @@ int stage_batch_args(cles_ctx *c, int in, int out, int istage, const double coef
     default: tile_counts<TileA>(a.g, a); break;
   }
   memcpy(dst, &a, sizeof(a));
-  return a.ntx * a.nty * a.ntz;
+  return 8 * a.per_xcd;
 }
 
-hipError_t launch_stage_batch(int tile, int fp32, int visc, const void *tab, const int *tstart, int nreq, int nblocks,
-                              hipStream_t s) {
+#define CLES_TILE_SWITCH(tile, CALL)          \
+  switch (tile) {                             \
+    case 1: return CALL(TileB);               \
+    case 2: return CALL(TileC);               \
+    case 3: return CALL(TileD);               \
+    case 4: return CALL(TileE);               \
+    case 5: return CALL(TileF);               \
+    case 6: return CALL(TileG);               \
+    case 7: return CALL(TileH);               \
+    default: return CALL(TileA);              \
+  }
+
+hipError_t launch_stage_persist(int tile, int fp32, int visc, int les, const PersistArgs &a, hipStream_t s) {
+#define CALL(TL) persist_shape<TL>(fp32, visc, les, a, s)
+  if (les) tile = 7;
+  CLES_TILE_SWITCH(tile, CALL)
+#undef CALL
+}
+
+int persist_blocks(int tile, int fp32, int visc, int les) {
+#define CALL(TL) persist_blocks_shape<TL>(fp32, visc, les)
+  if (les) tile = 7;
+  CLES_TILE_SWITCH(tile, CALL)
+#undef CALL
+}
+
+int stage_tiles(cles_ctx *c, int tile) {
+  StageArgs a;
+  a.g = c->g;
+#define CALL(TL) (tile_counts<TL>(a.g, a), a.ntx * a.nty * a.ntz)
+  CLES_TILE_SWITCH(tile, CALL)
+#undef CALL
+}
+
+hipError_t launch_stage_batch(int tile, int fp32, int visc, int les, const void *tab, const int *tstart, int nreq,
+                              int nblocks, hipStream_t s) {
   const StageArgs *t = (const StageArgs *)tab;
   switch (tile) {
-    case 1: return launch_batch_shape<TileB>(fp32, visc, t, tstart, nreq, nblocks, s);
-    case 2: return launch_batch_shape<TileC>(fp32, visc, t, tstart, nreq, nblocks, s);
-    case 3: return launch_batch_shape<TileD>(fp32, visc, t, tstart, nreq, nblocks, s);
-    case 4: return launch_batch_shape<TileE>(fp32, visc, t, tstart, nreq, nblocks, s);
-    case 5: return launch_batch_shape<TileF>(fp32, visc, t, tstart, nreq, nblocks, s);
-    case 6: return launch_batch_shape<TileG>(fp32, visc, t, tstart, nreq, nblocks, s);
-    case 7: return launch_batch_shape<TileH>(fp32, visc, t, tstart, nreq, nblocks, s);
-    default: return launch_batch_shape<TileA>(fp32, visc, t, tstart, nreq, nblocks, s);
+    case 1: return launch_batch_shape<TileB>(fp32, visc, les, t, tstart, nreq, nblocks, s);
+    case 2: return launch_batch_shape<TileC>(fp32, visc, les, t, tstart, nreq, nblocks, s);
+    case 3: return launch_batch_shape<TileD>(fp32, visc, les, t, tstart, nreq, nblocks, s);
+    case 4: return launch_batch_shape<TileE>(fp32, visc, les, t, tstart, nreq, nblocks, s);
+    case 5: return launch_batch_shape<TileF>(fp32, visc, les, t, tstart, nreq, nblocks, s);
+    case 6: return launch_batch_shape<TileG>(fp32, visc, les, t, tstart, nreq, nblocks, s);
+    case 7: return launch_batch_shape<TileH>(fp32, visc, les, t, tstart, nreq, nblocks, s);
+    default: return launch_batch_shape<TileA>(fp32, visc, les, t, tstart, nreq, nblocks, s);
   }
 }
 

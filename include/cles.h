@@ -246,12 +246,21 @@ int cles_time_les(cles_ctx *c, int nrep, double *ms);
 int cles_integrals(cles_ctx *c, double out[6]);
 
 /* ---------------------------------------------------------------------------
- * Batched serving.  No counterpart in the reference: several independent
- * periodic solves (any size, Mach, Reynolds number; no LES model, no immersed
- * body; RK2 or RK3) are advanced together by batched stage and seam kernels
- * whose workgroups cover every member's tiles, captured once as a hipGraph
- * per step.  Members are ordinary contexts with their state uploaded; each
- * member's result equals what cles_advance gives for it alone.
+ * Batched serving.  No counterpart in the reference (its time loop,
+ * solvers.c:223-294, runs one grid): several independent periodic solves
+ * (any size, Mach, Reynolds number, precision, with or without the LES model
+ * and the shock sensor; no immersed body, no statistics; RK2 or RK3, one
+ * fluxorder per batch) are advanced together, one step captured as a
+ * hipGraph: the LES model and sensor of each member, then either batched
+ * stage and seam launches (default) or one persistent kernel per
+ * (precision, model) group that runs every stage of every member, its
+ * workgroups pulling tile and seam items from per-XCD queues and waiting on
+ * per-member stage counters (cles_batch_set_mode).  Members are contexts
+ * with their state uploaded; each member's result equals what cles_advance
+ * gives for it alone.  From cles_batch_launch to cles_batch_wait every call
+ * on a member returns CLES_ESTATE; a call that changes a member makes the
+ * next launch rebuild
+ * the batched step; cles_destroy of a member removes it from its batch.
  * ------------------------------------------------------------------------ */
 typedef struct cles_batch cles_batch;
 /* Device bytes cles_create allocates for p (the admission footprint). */
@@ -269,6 +278,21 @@ size_t cles_batch_used(const cles_batch *b);
  * (64 threads), 7 8x4x3 (128 threads); chosen by the autotuner of the
  * serving loop. */
 int cles_batch_set_tile(cles_batch *b, int tile);
+/* Stages of the batched step: one batched launch per group and stage plus
+ * one seam launch per stage (default, the faster form on gfx950), or one
+ * persistent kernel per group. */
+#define CLES_BATCH_PERSISTENT 0
+#define CLES_BATCH_LAUNCHES 1
+int cles_batch_set_mode(cles_batch *b, int mode);
+/* Polls of a stage dependency wait of the persistent kernel before it gives
+ * up (default 4194304, about a second): the kernel then stops and the next
+ * cles_batch_wait / cles_batch_advance returns CLES_EDEVICE, the member
+ * states undefined. */
+int cles_batch_set_spin_limit(cles_batch *b, unsigned polls);
+/* Replay the batched step as a captured hipGraph (on, the default) or launch
+ * every step's kernels on the batch stream (off: no capture and instantiation
+ * after a membership change, which pays where members come and go often). */
+int cles_batch_set_graph(cles_batch *b, int on);
 /* Advance every member by nsteps time steps (one graph replay per step) and
  * wait for them. */
 int cles_batch_advance(cles_batch *b, int nsteps);

@@ -1,7 +1,9 @@
 """Batched serving on the GPU: a member's result is the same alone and in a
-mix of other solves, whatever the tile shape, the batch composition or the
-step at which it joined; admission respects the HBM budget.  Results are
-compared bit for bit with the single-context path (cles_advance)."""
+mix of other solves, whatever the stage mode (persistent kernel or batched
+launches), the tile shape, the batch composition, the models of the members
+(LES, shock sensor) or the step at which it joined; changes to a member
+between launches are followed; admission respects the HBM budget.  Results
+are compared bit for bit with the single-context path (cles_advance)."""
 import numpy as np
 import pytest
 
@@ -11,12 +13,15 @@ from chimera_comp_les_amd import cases
 pytestmark = pytest.mark.gpu
 
 
-def request(kind, N, Ma, Re=None, prec=cl.PREC_FP32BULK):
+MODES = [cl.BATCH_PERSISTENT, cl.BATCH_LAUNCHES]
+
+
+def request(kind, N, Ma, Re=None, prec=cl.PREC_FP32BULK, les=0, **extra):
     if kind == "vortex":
         cfg, coor, Q = cases.vortex_case(N=N, mach=Ma)
     else:
-        cfg, coor, Q = cases.tgv_case(N=N, Re=Re, Ma=Ma, les=0)
-    return cfg, coor, Q, prec
+        cfg, coor, Q = cases.tgv_case(N=N, Re=Re, Ma=Ma, les=les)
+    return dict(cfg, **extra), coor, Q, prec
 
 
 def solo(req, steps):
@@ -49,12 +54,14 @@ def test_footprint_is_the_allocation():
         s.close()
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7])
-def test_mix_equals_solo(tile):
+def test_mix_equals_solo(tile, mode):
     reqs = [request(*m) for m in MIX]
     ref = [solo(r, 4) for r in reqs]
     b = cl.Batch(1 << 40)
     b.set_tile(tile)
+    b.set_mode(mode)
     ms = [member(r) for r in reqs]
     for s in ms:
         assert b.add(s)
@@ -68,9 +75,11 @@ def test_mix_equals_solo(tile):
         s.close()
 
 
-def test_join_midway_and_leave():
+@pytest.mark.parametrize("mode", MODES)
+def test_join_midway_and_leave(mode):
     reqs = [request(*m) for m in MIX[:3]]
     b = cl.Batch(1 << 40)
+    b.set_mode(mode)
     m0, m1 = member(reqs[0]), member(reqs[1])
     assert b.add(m0) and b.add(m1)
     b.advance(2)
@@ -104,14 +113,17 @@ def test_admission_by_footprint():
 
 
 def test_rejects_unsupported_members():
-    cfg, coor, Q = cases.tgv_case(N=16, les=2)
-    s = cl.Solver(cfg, coor)
-    s.set_state(Q)
+    """statistics (Do_averaging) and non-periodic grids stay solo"""
     b = cl.Batch(1 << 40)
-    with pytest.raises(cl.CLESError):
-        b.add(s)
+    for cfg, coor, Q in (
+            (lambda c: (dict(c[0], averaging=1), c[1], c[2]))(cases.tgv_case(N=16, les=0)),
+            (lambda c: (dict(c[0], bctype=(1, 1, 7, 7, 7, 7)), c[1], c[2]))(cases.tgv_case(N=16, les=0))):
+        s = cl.Solver(cfg, coor)
+        s.set_state(Q)
+        with pytest.raises(cl.CLESError):
+            b.add(s)
+        s.close()
     b.close()
-    s.close()
 
 
 @pytest.mark.parametrize("prec", [cl.PREC_FP32BULK, cl.PREC_FP64])
@@ -122,9 +134,10 @@ def test_large_viscous_member_equals_solo(prec):
     it once showed here as batch/solo differences of 1e-4 at 128^3.)"""
     req = request("tgv", 64, 0.3, 1000.0, prec)
     ref = solo(req, 10)
-    for tile in (7, 4):
+    for tile, mode in ((7, cl.BATCH_PERSISTENT), (4, cl.BATCH_PERSISTENT), (7, cl.BATCH_LAUNCHES)):
         b = cl.Batch(1 << 40)
         b.set_tile(tile)
+        b.set_mode(mode)
         s = member(req)
         assert b.add(s)
         b.advance(10)
@@ -162,3 +175,123 @@ def test_two_d_members_equal_solo():
         du = np.abs(outs[k][0][..., 1] - Q0[..., 1]).max()
         assert du <= 1e-14 * np.abs(Q0[..., 1]).max(), du
         assert np.abs(outs[k][0][..., 0] - Q0[..., 0]).max() > 1e-8
+
+
+def solo_les(req, steps):
+    cfg, coor, Q, prec = req
+    s = cl.Solver(cfg, coor, precision=prec)
+    s.set_state(Q)
+    s.advance(steps)
+    out = s.state() + s.les_fields()[:2]
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("mode", MODES)
+def test_les_and_sensor_members_equal_solo(mode, graph):
+    """Members with the dynamic Smagorinsky model (FP64 and FP32-bulk, the
+    homogeneous-direction form, constant Cs) and with the shock sensor,
+    batched with plain members: state, Cs and nu_t equal the solo run bit
+    for bit after an odd and an even number of steps (the Cs buffers swap
+    every step)."""
+    reqs = [request("tgv", 20, 0.3, 800.0, cl.PREC_FP64, les=2),
+            request("tgv", 16, 0.2, 500.0, cl.PREC_FP32BULK, les=2),
+            request("tgv", 18, 0.3, 900.0, cl.PREC_FP64, les=2, i_homo_filter=1, k_homo_filter=1),
+            request("tgv", 16, 0.4, 700.0, cl.PREC_FP64, les=1),
+            request("tgv", 18, 0.3, 600.0, cl.PREC_FP64, les=2, calc_sigma=1),
+            request("vortex", 20, 0.3),
+            request("tgv", 16, 0.2, 300.0, cl.PREC_FP32BULK)]
+    b = cl.Batch(1 << 40)
+    b.set_mode(mode)
+    b.set_graph(graph)
+    ms = [member(r) for r in reqs]
+    for s in ms:
+        assert b.add(s)
+    done = 0
+    for k in (3, 2):
+        b.advance(k)
+        done += k
+        for i, (s, r) in enumerate(zip(ms, reqs)):
+            ref = solo_les(r, done)
+            got = s.state() + s.les_fields()[:2]
+            for a, e in zip(got, ref):
+                assert np.array_equal(a, e), (i, done)
+    b.close()
+    for s in ms:
+        s.close()
+
+
+@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("mode", MODES)
+def test_member_changes_between_launches(mode, graph):
+    """A solo advance, a new state and a time reset on a member between two
+    batched launches are followed: every member ends where the solo path
+    given the same calls ends."""
+    reqs = [request("tgv", 20, 0.3, 800.0, cl.PREC_FP64, les=2),
+            request("vortex", 16, 0.4, prec=cl.PREC_FP32BULK),
+            request("tgv", 16, 0.2, 400.0, cl.PREC_FP32BULK)]
+    b = cl.Batch(1 << 40)
+    b.set_mode(mode)
+    b.set_graph(graph)
+    ms = [member(r) for r in reqs]
+    for s in ms:
+        assert b.add(s)
+    b.advance(2)
+    ms[0].advance(1)                 # solo step of a member (rotates its buffers)
+    ms[1].set_state(reqs[1][2])      # FP32-bulk member restarts from its initial state
+    b.advance(3)
+    ref0 = solo_les(reqs[0], 6)
+    got0 = ms[0].state() + ms[0].les_fields()[:2]
+    for a, e in zip(got0, ref0):
+        assert np.array_equal(a, e)
+    assert np.array_equal(ms[1].state()[0], solo(reqs[1], 3)[0])
+    assert np.array_equal(ms[2].state()[0], solo(reqs[2], 5)[0])
+    assert [s.ti for s in ms] == [6, 5, 5]
+    # a member closed while in the batch leaves it; the others go on
+    ms[1].close()
+    assert b.size() == 2
+    b.advance(1)
+    assert np.array_equal(ms[2].state()[0], solo(reqs[2], 6)[0])
+    b.close()
+    for s in (ms[0], ms[2]):
+        s.close()
+
+
+def test_member_calls_refused_while_running():
+    req = request("tgv", 48, 0.3, 800.0, cl.PREC_FP64)
+    b = cl.Batch(1 << 40)
+    s = member(req)
+    assert b.add(s)
+    b.launch(50)
+    with pytest.raises(cl.CLESError, match="running batch"):
+        s.advance(1)
+    b.wait()
+    s.advance(1)                     # idle again: a solo step is fine
+    assert s.ti == 51
+    b.close()
+    s.close()
+
+
+def test_persistent_wait_gives_up_and_recovers():
+    """The dependency wait of the persistent kernel is bounded: with a limit
+    of one poll a stage-1 item finds stage 0 unfinished, every workgroup
+    stops and the wait reports the failure; with the default limit the next
+    steps match the solo path again."""
+    req = request("tgv", 96, 0.3, 800.0, cl.PREC_FP32BULK)
+    b = cl.Batch(1 << 40)
+    b.set_mode(cl.BATCH_PERSISTENT)
+    s = member(req)
+    assert b.add(s)
+    b.set_spin_limit(1)
+    with pytest.raises(cl.CLESError, match="gave up"):
+        b.advance(1)
+    b.set_spin_limit(1 << 22)
+    s.set_state(req[2])
+    s.ti = 0
+    b.advance(3)
+    Q, P = s.state()
+    ref = solo(req, 3)
+    assert np.array_equal(Q, ref[0]) and np.array_equal(P, ref[1])
+    b.close()
+    s.close()

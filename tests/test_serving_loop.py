@@ -1,7 +1,8 @@
 """Host logic of the continuous-batching request loop (serving.py) on the CPU,
 with stand-in contexts and batch: every request runs exactly its steps, leaves
 when done, admission respects the HBM budget and the in-flight cap, and the
-smallest-work-first policy serves short requests before long ones."""
+smallest-work-first policy serves short requests before long ones, and
+aging lets a long request that has waited past the limit go first."""
 import pytest
 
 import chimera_comp_les_amd as cl
@@ -35,6 +36,12 @@ class FakeBatch:
         FakeBatch.log = []
 
     def set_tile(self, t):
+        pass
+
+    def set_mode(self, m):
+        pass
+
+    def set_graph(self, on):
         pass
 
     def used(self):
@@ -84,7 +91,7 @@ def payloads_of(reqs):
     return {r.rid: (r, None, None) for r in reqs}
 
 
-@pytest.mark.parametrize("policy", ["sjf", "all"])
+@pytest.mark.parametrize("policy", ["aging", "sjf", "all"])
 def test_every_request_runs_exactly_its_steps(fake, monkeypatch, policy):
     reqs, _ = requests()
     pay = payloads_of(reqs)
@@ -145,3 +152,39 @@ def test_sequential_latency_is_a_fifo_server(monkeypatch):
     assert out["latency_s"] == {0: 2.0, 1: 2.5, 2: 1.0}
     assert out["digests"] == {0: (0, 5), 1: (1, 5), 2: (2, 5)}
     assert out["wall_s"] == 4.0
+
+
+def test_admission_order_ages_long_requests():
+    R = sv.Request
+    waiting = [R(0, "tgv", 128, 0.3, 100.0, 10, 0.0), R(1, "tgv", 32, 0.3, 100.0, 10, 4.0),
+               R(2, "tgv", 64, 0.3, 100.0, 10, 1.0), R(3, "tgv", 96, 0.3, 100.0, 10, 0.5)]
+    ids = lambda rs: [r.rid for r in rs]
+    assert ids(sv.admission_order(waiting, 4.5, "sjf", 2.0)) == [1, 2, 3, 0]
+    # at t = 4.5, requests 0 (waited 4.5 s), 3 (4.0 s) and 2 (3.5 s) are past 2 s: arrival order first
+    assert ids(sv.admission_order(waiting, 4.5, "aging", 2.0)) == [0, 3, 2, 1]
+    assert ids(sv.admission_order(waiting, 1.2, "aging", 2.0)) == [1, 2, 3, 0]
+    assert ids(sv.admission_order(waiting, 4.5, "all", 2.0)) == [0, 3, 2, 1]
+
+
+def test_aging_bounds_the_wait_of_a_long_request(fake, monkeypatch):
+    """A 128^3 request among a steady stream of small ones: under "sjf" with a
+    cap that fits one large or many small members it waits until the small
+    ones run out; under "aging" it starts once it has waited age_s."""
+    clock = [0.0]
+
+    def tick():
+        clock[0] += 0.01
+        return clock[0]
+
+    monkeypatch.setattr(sv.time, "perf_counter", tick)
+    monkeypatch.setattr(sv.time, "sleep", lambda d: None)
+    monkeypatch.setattr(cl, "footprint", lambda r, prec=None: footprint_of(r.N))
+    R = sv.Request
+    reqs = [R(0, "tgv", 128, 0.3, 100.0, 20, 0.1)] + [R(q, "tgv", 32, 0.3, 100.0, 20, 0.02 * q) for q in range(1, 200)]
+    pay = payloads_of(reqs)
+    start = {}
+    for policy in ("sjf", "aging"):
+        sv.run_batched(reqs, pay, budget=1 << 40, tile=1, chunk=5, policy=policy, cap_cells=(128 ** 3) // 2,
+                       age_s=0.5)
+        start[policy] = next(i for i, m in enumerate(FakeBatch.log) if any(r.rid == 0 for r in m))
+    assert start["aging"] < start["sjf"] / 4, start
