@@ -109,6 +109,20 @@ __device__ __forceinline__ void load6(const StageArgs &a, long long n, T s[6]) {
   s[4] = T(a.q[n + 4 * np]);
   s[5] = T(a.p[n]);
 }
+// 32-bit byte offsets of a node in a plane (a plane holds fewer than 2^29
+// nodes, checked at cles_create): loads and stores address a uniform plane
+// base plus the offset, the scalar-base form of the global memory
+// instructions, instead of forming a 64-bit address per access.
+__device__ __forceinline__ unsigned node_off(const Grid &g, int k, int j, int i) {
+  return ((unsigned)k * (unsigned)g.sz + (unsigned)j * (unsigned)g.sy + (unsigned)i) * 8u;
+}
+__device__ __forceinline__ double ldo(const double *plane, unsigned off) {
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(plane) + off);
+}
+__device__ __forceinline__ void sto(double *plane, unsigned off, double v) {
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(plane) + off) = v;
+}
+
 // Node data of the current face direction staged in LDS by the workgroup:
 // S[c * ns + o], c = 0..6 for rho, rhoU, rhoV, rhoW, rhoE, p and the
 // contravariant velocity of the direction; o0 is the slot of the face's lower
@@ -352,19 +366,25 @@ __device__ void conv_face_plain(const StageArgs &a, const Sv &sv, int d, int k, 
   }
 }
 
-template <typename T, int SCH>
-__device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]);
+// CEN: sigma = 1 on every face (no -calc_sigma, s_x = s_y = s_z = 1), where
+// the blend reduces to the central flux and the upwind part is not compiled
+template <typename T, int SCH, bool CEN = false>
+__device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5],
+                              const double *hp);
 
 // FP32-bulk four-point central flux (sigma = 1).  fp_2(c, U) = c * fp4(U) for
 // a constant c, so rho, rhoE + p and the pressure are split into reference
 // parts, whose fluxes rho_ref fp4(U), h_ref fp4(U) and p_ref h use the
-// contravariant velocity in double, and float deviations for the rest.
-__device__ bool conv_face_les_split(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]) {
+// contravariant velocity in double, and float deviations for the rest.  hp:
+// the face's metric averages when the caller has them, else null.
+template <bool CEN>
+__device__ bool conv_face_les_split(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5],
+                                    const double *hp) {
   const Grid &g = a.g;
   const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
   const int id = d == 0 ? i : d == 1 ? j : k;
   if (!(id > 0 && id < m - 2 && (a.central == 2 || a.central == 3))) return false;
-  if (sigma_at(a, d, k, j, i) != 1.0) return false;
+  if (!CEN && sigma_at(a, d, k, j, i) != 1.0) return false;
   const long long s = d == 0 ? 1 : d == 1 ? g.sy : g.sz;
   const long long n0 = nid(g, k, j, i), n1 = n0 + s, nm = n0 - s, n2 = n0 + 2 * s;
   const double sum = d == 0 ? nvat(a, n0) + nvat(a, n1) + nvat(a, n2) + nvat(a, nm)
@@ -373,7 +393,13 @@ __device__ bool conv_face_les_split(const StageArgs &a, const Sv &sv, int d, int
   const long long np = g.np;
   const double *mt = metric(a, d);
   double hd[3];
-  face_metric(a, d, n0, n1, hd);
+  if (hp) {
+    hd[0] = hp[0];
+    hd[1] = hp[1];
+    hd[2] = hp[2];
+  } else {
+    face_metric(a, d, n0, n1, hd);
+  }
   float r[4], u[4], v[4], w[4], ep[4], pp[4], U[4];
   double Ud[4];
   const long long nn[4] = {nm, n0, n1, n2};
@@ -426,12 +452,15 @@ __device__ bool conv_face_les_split(const StageArgs &a, const Sv &sv, int d, int
 // formConvectionLES face (rhs.c:3466-3706, 3714-3937, 3947-4168).  The
 // FP32-bulk form uses the split central flux above and, for the rare faces
 // it does not cover (boundaries, immersed nodes, sigma < 1), this function in
-// double.  SCH 1: the upwind part may use the weno5 / roe schemes.
-template <typename T, int SCH>
-__device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5]) {
+// double.  SCH 1: the upwind part may use the weno5 / roe schemes.  hp: the
+// face's metric averages when the caller has them, else null.
+template <typename T, int SCH, bool CEN>
+__device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, int j, int i, double F[5],
+                              const double *hp) {
   if constexpr (sizeof(T) == 4) {
     // the double fallback reads global memory: the staged velocity is the float form's
-    if (!conv_face_les_split(a, sv, d, k, j, i, F)) conv_face_les<double, 0>(a, Sv{nullptr, 0, 0, 0, false}, d, k, j, i, F);
+    if (!conv_face_les_split<CEN>(a, sv, d, k, j, i, F, hp))
+      conv_face_les<double, 0, CEN>(a, Sv{nullptr, 0, 0, 0, false}, d, k, j, i, F, hp);
     return;
   }
   const Grid &g = a.g;
@@ -441,7 +470,13 @@ __device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, in
   const long long n0 = nid(g, k, j, i), n1 = n0 + s;
   const long long np = g.np;
   T h[3];
-  face_metric(a, d, n0, n1, h);
+  if (hp) {
+    h[0] = T(hp[0]);
+    h[1] = T(hp[1]);
+    h[2] = T(hp[2]);
+  } else {
+    face_metric(a, d, n0, n1, h);
+  }
   T Fc[5];
   bool four = false;
   if (id > 0 && id < m - 2 && (a.central == 2 || a.central == 3)) {
@@ -509,6 +544,11 @@ __device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, in
     Fc[3] = wc * Uc + pc * h[2];
     Fc[4] = Uc * (ec + pc);
   }
+  if constexpr (CEN) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) F[q] = Fc[q];
+    return;
+  }
   const T sigma = T(sigma_at(a, d, k, j, i));
   if (sigma == T(1)) {
     // (1 - sigma) * F_upwind == 0 for every finite upwind flux: the blend of
@@ -556,7 +596,10 @@ __device__ void conv_face_les(const StageArgs &a, const Sv &sv, int d, int k, in
 }
 
 // instantiate the double version used by the float fallback
-template __device__ void conv_face_les<double, 0>(const StageArgs &, const Sv &, int, int, int, int, double *);
+template __device__ void conv_face_les<double, 0, false>(const StageArgs &, const Sv &, int, int, int, int,
+                                                         double *, const double *);
+template __device__ void conv_face_les<double, 0, true>(const StageArgs &, const Sv &, int, int, int, int,
+                                                        double *, const double *);
 
 // ------------------------------ viscous ----------------------------------
 struct Win {  // LDS window of primitives with a one-node halo
@@ -638,12 +681,52 @@ __device__ __forceinline__ void visc_load(const StageArgs &a, int d, int k, int 
   }
 }
 
-// formViscous face (rhs.c:1487-1707, 1719-1918, 1926-2121); G[0..3] =
-// rhoU..rhoE flux, al = LES stress (alpha) of the face; v holds the face's
-// global-memory inputs (visc_load).
+// Viscous coefficients of a face from its densities, pressures and LES
+// inputs (rhs.c:1616-1635, 1681-1697): cf[0] = ajc nu, cf[1] = ajc nu_t rho
+// (the SGS stress factor), cf[2] = the factor of S2 in the isotropic SGS
+// term.  ajc: the face's aj average; fin: nu_t at A and B, the filter width
+// at A, rho at A + 1.
+template <typename T, bool LES, bool CEN>
+__device__ __forceinline__ void visc_coef(const StageArgs &a, int d, int k, int j, int i, double ajd, double rA,
+                                          double rB, double pA, double pB, const double fin[4], double cf[3]) {
+  const T ajc = T(ajd);
+  T nu = T(a.inv_ren);
+  if (LES) nu = T(a.inv_ren * 0.5 * (rA + fin[3]));  // rho at [i] and [i+1] on every face
+  if (a.suth) {
+    // Sutherland (rhs.c:1628-1635): Tr = gamma Ma^2 p_face / rho_face,
+    // nu = 1/ren (1 + C/Tref) / (Tr + C/Tref) Tr^1.5
+    const double Tr = a.gm2 * (pA + pB) * rcp64(rA + rB);
+    nu = Tr < 0 ? T(a.inv_ren) : T(a.suth_a * rcp64(Tr + a.suth_b) * (Tr * sqrt(Tr)));
+  }
+  cf[0] = double(ajc * nu);
+  if (LES) {
+    const Grid &g = a.g;
+    const int id = d == 0 ? i : d == 1 ? j : k;
+    const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
+    constexpr double kIso = 0.06 * 2.0 / 3.0;  // 2/3 CI, CI = 0.06 (rhs.c:13)
+    const T sA = CEN ? T(1) : T(sigma_at(a, d, k, j, i));
+    const T sB = CEN ? T(1) : T(sigma_at(a, d, k + (d == 2), j + (d == 1), i + (d == 0)));
+    T nu_t = T(0.5) * (T(fin[0]) * sA + T(fin[1]) * sB);
+    const T rho_c = T(0.5 * (rA + rB));
+    const T sigma_c = T(0.5) * (sA + sB);
+    if ((a.bc[2 * d] == 1 && id == 0) || (a.bc[2 * d + 1] == 1 && id == m - 2)) nu_t = T(0);
+    cf[1] = double(ajc * (nu_t * rho_c));
+    // isotropic part -2/3 CI rho Delta^2 S2 sigma, without the face Jacobian (rhs.c:1693-1697)
+    cf[2] = double(T(fin[2]) * T(kIso) * rho_c * sigma_c);
+  }
+}
+
+// formViscous face (rhs.c:1487-1707, 1719-1918, 1926-2121) from the LDS
+// window and the face data fm[c * fs]: c = 0..8 the metric averages (csi,
+// eta, zet), 9..11 the coefficients of visc_coef.  With r_ab = du_a/dx_b
+// (the face's velocity gradient through its metrics) the reference's
+// contraction g_t1 du_a/dt + r_ba m_b - div m_a equals
+// m_b (r_ab + r_ba) - div m_a, and its divergence term 2/3 (r_11 + r_22 +
+// r_33); the heat flux g_t1 dT/dt is m_b (dT/dt m_t,b).  G[0..3] = rhoU..rhoE
+// flux, al = LES stress (alpha) of the face.
 template <class TL, typename T, bool LES>
-__device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d, int k, int j, int i, const VisIn &v,
-                          T G[4], T al[3]) {
+__device__ void visc_face_s(const StageArgs &a, const T *W, const Win &win, int d, int k, int j, int i,
+                            const double *fm, int fs, T G[4], T al[3]) {
   constexpr int WX = TL::WX, WY = TL::WY, NW = TL::NW;
   const Grid &g = a.g;
   const long long gs[3] = {1, g.sy, g.sz};
@@ -651,7 +734,7 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
   const long long gA = nid(g, k, j, i), gB = gA + gs[d];
   const int wA = widx<TL>(win, k, j, i), wB = wA + ws[d];
   const T *U = W, *V = W + NW, *Wz = W + 2 * NW, *Tm = W + 3 * NW;
-  T dd[3][3], dT[3];
+  T dd[3][3], dT[3];  // dd[t][a]: d u_a / d(computational direction t)
   dd[d][0] = U[wB] - U[wA];
   dd[d][1] = V[wB] - V[wA];
   dd[d][2] = Wz[wB] - Wz[wA];
@@ -661,59 +744,23 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
     if (t == d) continue;
     cross_deriv<TL>(a, W, wA, wB, ws[t], gA, gB, gs[t], d == 0 && t == 2, dd[t], dT[t]);
   }
-  T csi0 = T(0.5 * (v.m[0][0] + v.m[0][1])), csi1 = T(0.5 * (v.m[1][0] + v.m[1][1])),
-    csi2 = T(0.5 * (v.m[2][0] + v.m[2][1]));
-  T eta0 = T(0.5 * (v.m[3][0] + v.m[3][1])), eta1 = T(0.5 * (v.m[4][0] + v.m[4][1])),
-    eta2 = T(0.5 * (v.m[5][0] + v.m[5][1]));
-  T zet0 = T(0.5 * (v.m[6][0] + v.m[6][1])), zet1 = T(0.5 * (v.m[7][0] + v.m[7][1])),
-    zet2 = T(0.5 * (v.m[8][0] + v.m[8][1]));
-  T m0, m1, m2, g11, g21, g31;
-  if (d == 0) {
-    m0 = csi0; m1 = csi1; m2 = csi2;
-    g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2;
-    g21 = eta0 * csi0 + eta1 * csi1 + eta2 * csi2;
-    g31 = zet0 * csi0 + zet1 * csi1 + zet2 * csi2;
-  } else if (d == 1) {
-    m0 = eta0; m1 = eta1; m2 = eta2;
-    g11 = csi0 * eta0 + csi1 * eta1 + csi2 * eta2;
-    g21 = eta0 * eta0 + eta1 * eta1 + eta2 * eta2;
-    g31 = zet0 * eta0 + zet1 * eta1 + zet2 * eta2;
-  } else {
-    m0 = zet0; m1 = zet1; m2 = zet2;
-    g11 = csi0 * zet0 + csi1 * zet1 + csi2 * zet2;
-    g21 = eta0 * zet0 + eta1 * zet1 + eta2 * zet2;
-    g31 = zet0 * zet0 + zet1 * zet1 + zet2 * zet2;
-  }
-  const T dudc = dd[0][0], dvdc = dd[0][1], dwdc = dd[0][2];
-  const T dude = dd[1][0], dvde = dd[1][1], dwde = dd[1][2];
-  const T dudz = dd[2][0], dvdz = dd[2][1], dwdz = dd[2][2];
-  T r11 = dudc * csi0 + dude * eta0 + dudz * zet0;
-  T r21 = dvdc * csi0 + dvde * eta0 + dvdz * zet0;
-  T r31 = dwdc * csi0 + dwde * eta0 + dwdz * zet0;
-  T r12 = dudc * csi1 + dude * eta1 + dudz * zet1;
-  T r22 = dvdc * csi1 + dvde * eta1 + dvdz * zet1;
-  T r32 = dwdc * csi1 + dwde * eta1 + dwdz * zet1;
-  T r13 = dudc * csi2 + dude * eta2 + dudz * zet2;
-  T r23 = dvdc * csi2 + dvde * eta2 + dvdz * zet2;
-  T r33 = dwdc * csi2 + dwde * eta2 + dwdz * zet2;
-  T ajc = T(0.5 * (v.aj[0] + v.aj[1]));
-  const double rA = v.r[0], rB = v.r[1];
-  T nu = T(a.inv_ren);
-  if (LES) nu = T(a.inv_ren * 0.5 * (rA + v.rA1));  // rho at [i] and [i+1] on every face
-  if (a.suth) {
-    // Sutherland (rhs.c:1628-1635): Tr = gamma Ma^2 p_face / rho_face,
-    // nu = 1/ren (1 + C/Tref) / (Tr + C/Tref) Tr^1.5
-    const double Tr = a.gm2 * (v.p[0] + v.p[1]) * rcp64(rA + rB);
-    nu = Tr < 0 ? T(a.inv_ren) : T(a.suth_a * rcp64(Tr + a.suth_b) * (Tr * sqrt(Tr)));
-  }
-  T div = T(2. / 3.) * (dudc * csi0 + dude * eta0 + dudz * zet0 + dvdc * csi1 + dvde * eta1 + dvdz * zet1 +
-                        dwdc * csi2 + dwde * eta2 + dwdz * zet2);
-  // deviatoric stress contractions, shared by the molecular and the SGS
-  // stress: (g . grad u + r . m - div m) of rhs.c:1646-1655, 1674-1679
-  const T bu = g11 * dudc + g21 * dude + g31 * dudz + r11 * m0 + r21 * m1 + r31 * m2 - div * m0;
-  const T bv = g11 * dvdc + g21 * dvde + g31 * dvdz + r12 * m0 + r22 * m1 + r32 * m2 - div * m1;
-  const T bw = g11 * dwdc + g21 * dwde + g31 * dwdz + r13 * m0 + r23 * m1 + r33 * m2 - div * m2;
-  const T cm = ajc * nu;
+  T mt[3][3];  // mt[t][b]: face metric row t (csi, eta, zet), component b
+#pragma unroll
+  for (int t = 0; t < 3; t++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) mt[t][b] = T(fm[(3 * t + b) * fs]);
+  const T m0 = mt[d][0], m1 = mt[d][1], m2 = mt[d][2];
+  T r[3][3];
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) r[p][b] = dd[0][p] * mt[0][b] + dd[1][p] * mt[1][b] + dd[2][p] * mt[2][b];
+  const T div = T(2. / 3.) * (r[0][0] + r[1][1] + r[2][2]);
+  const T s01 = r[0][1] + r[1][0], s02 = r[0][2] + r[2][0], s12 = r[1][2] + r[2][1];
+  const T bu = m0 * (r[0][0] + r[0][0] - div) + m1 * s01 + m2 * s02;
+  const T bv = m0 * s01 + m1 * (r[1][1] + r[1][1] - div) + m2 * s12;
+  const T bw = m0 * s02 + m1 * s12 + m2 * (r[2][2] + r[2][2] - div);
+  const T cm = T(fm[9 * fs]);
   T fu = bu * cm, fv = bv * cm, fw = bw * cm;
   const int id = d == 0 ? i : d == 1 ? j : k;
   const int m = d == 0 ? g.mx : d == 1 ? g.my : g.mz;
@@ -728,34 +775,42 @@ __device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d,
     uh2 = T(0.5) * (Wz[wB] + Wz[wA]);
   }
   // heat flux with the conductivity gamma nu / (Pr (gamma - 1)) (rhs.c:1666-1670)
-  T fe = (fu * uh0 + fv * uh1 + fw * uh2) + (g11 * dT[0] + g21 * dT[1] + g31 * dT[2]) * (cm * T(a.kheat));
+  T gT = T(0);
+#pragma unroll
+  for (int b = 0; b < 3; b++) gT += mt[d][b] * (dT[0] * mt[0][b] + dT[1] * mt[1][b] + dT[2] * mt[2][b]);
+  const T fe = (fu * uh0 + fv * uh1 + fw * uh2) + gT * (cm * T(a.kheat));
   if (LES) {
-    constexpr double kIso = 0.06 * 2.0 / 3.0;  // 2/3 CI, CI = 0.06 (rhs.c:13)
-    T sA = T(sigma_at(a, d, k, j, i));
-    T sB = T(sigma_at(a, d, k + (d == 2), j + (d == 1), i + (d == 0)));
-    T nu_t = T(0.5) * (T(v.nut[0]) * sA + T(v.nut[1]) * sB);
-    T rho_c = T(0.5 * (rA + rB));
-    T sigma_c = T(0.5) * (sA + sB);
-    if ((a.bc[2 * d] == 1 && id == 0) || (a.bc[2 * d + 1] == 1 && id == m - 2)) nu_t = T(0);
-    const T ct = ajc * (nu_t * rho_c);
-    T tx = bu * ct, ty = bv * ct, tz = bw * ct;
-    T S2 = T(.5) * ((r11 - div) * (r11 - div) + r12 * r12 + r13 * r13 + r21 * r21 + (r22 - div) * (r22 - div) +
-                    r23 * r23 + r31 * r31 + r32 * r32 + (r33 - div) * (r33 - div));
-    T f2 = T(v.f2);  // (pow(1/aj, 1/3))^2 at the face's lower node
-    // isotropic part -2/3 CI rho Delta^2 S2 sigma, without the face Jacobian (rhs.c:1693-1697)
-    const T ss = S2 * f2 * T(kIso) * rho_c * sigma_c;
-    T Ssx = -ss * m0, Ssy = -ss * m1, Ssz = -ss * m2;
-    fu += tx + Ssx;
-    fv += ty + Ssy;
-    fw += tz + Ssz;
-    al[0] = tx + Ssx;
-    al[1] = ty + Ssy;
-    al[2] = tz + Ssz;
+    const T ct = T(fm[10 * fs]);
+    const T tx = bu * ct, ty = bv * ct, tz = bw * ct;
+    const T S2 = T(.5) * ((r[0][0] - div) * (r[0][0] - div) + r[0][1] * r[0][1] + r[0][2] * r[0][2] +
+                          r[1][0] * r[1][0] + (r[1][1] - div) * (r[1][1] - div) + r[1][2] * r[1][2] +
+                          r[2][0] * r[2][0] + r[2][1] * r[2][1] + (r[2][2] - div) * (r[2][2] - div));
+    const T ss = S2 * T(fm[11 * fs]);
+    al[0] = tx - ss * m0;
+    al[1] = ty - ss * m1;
+    al[2] = tz - ss * m2;
+    fu += al[0];
+    fv += al[1];
+    fw += al[2];
   }
   G[0] = fu;
   G[1] = fv;
   G[2] = fw;
   G[3] = fe;
+}
+
+// formViscous face from its global-memory inputs v (visc_load): the same
+// arithmetic as the staged form, visc_coef then visc_face_s.
+template <class TL, typename T, bool LES>
+__device__ void visc_face(const StageArgs &a, const T *W, const Win &win, int d, int k, int j, int i, const VisIn &v,
+                          T G[4], T al[3]) {
+  double fa[12];
+#pragma unroll
+  for (int c = 0; c < 9; c++) fa[c] = 0.5 * (v.m[c][0] + v.m[c][1]);
+  const double fin[4] = {LES ? v.nut[0] : 0.0, LES ? v.nut[1] : 0.0, LES ? v.f2 : 0.0, LES ? v.rA1 : 0.0};
+  visc_coef<T, LES, false>(a, d, k, j, i, 0.5 * (v.aj[0] + v.aj[1]), v.r[0], v.r[1], a.suth ? v.p[0] : 0.0,
+                           a.suth ? v.p[1] : 0.0, fin, fa + 9);
+  visc_face_s<TL, T, LES>(a, W, win, d, k, j, i, fa, 1, G, al);
 }
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
@@ -791,7 +846,7 @@ template <class TL> __device__ __forceinline__ int fk_of(int lx, int ly, int lz)
 }
 
 // One workgroup's tile of one grid; b only spreads the norm atomics.
-template <class TL, typename T, bool LES, bool VISC, int MODE, int SCH = 0>
+template <class TL, typename T, bool LES, bool VISC, int MODE, int SCH = 0, bool CEN = false>
 __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) {
   constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, WX = TL::WX, WY = TL::WY, NW = TL::NW;
   // The convective and viscous flux of a face are computed in one pass (one
@@ -809,14 +864,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   constexpr int NCH = LES ? 7 : 6;
   constexpr int NS0 = (TX + 4) * TY * TZ, NS1 = TX * (TY + 4) * TZ, NS2 = TX * TY * (TZ + 4);
   constexpr int NSM = NS0 > NS1 ? (NS0 > NS2 ? NS0 : NS2) : (NS1 > NS2 ? NS1 : NS2);
-  // staging pays where it saves the per-stencil-node division of the
-  // central flux (LES forms) and two workgroups still fit in a CU's LDS;
-  // elsewhere the extra LDS costs more occupancy than the loads it saves
-  constexpr bool STG = LES &&
-                       8 * (FC * NFD + NCH * NSM) + (VISC ? (int)sizeof(T) * 4 * NW : 0) <= 80 * 1024;
-  __shared__ double Fs[FC][NFD];
+  // Staged form (LES, every face of a direction in one pass of the
+  // workgroup): per direction, one load phase fills the node box (state, p,
+  // contravariant velocity) and the face buffer FM (the face's nine metric
+  // averages and aj, later its viscous coefficients), so the face passes read
+  // LDS only; the flux buffer Fs then reuses the box's memory.  Elsewhere
+  // (larger tiles) the face routines read global memory.
+  constexpr bool ONEPASS = TL::NFI <= TL::NT && TL::NFJ <= TL::NT && TL::NFK <= TL::NT;
+  constexpr int NFM = VISC ? 12 : 3;
+  constexpr int NSF = NCH * NSM > FC * NFD ? NCH * NSM : FC * NFD;
+  constexpr bool STG = LES && ONEPASS && 8 * NSF + (VISC ? (int)sizeof(T) * 4 * NW : 0) <= 160 * 1024 * TL::NT / 512;
+  __shared__ double SF[STG ? NSF : FC * NFD];
   __shared__ T Ws[VISC ? 4 * NW : 1];
-  __shared__ double Ss[STG ? NCH * NSM : 1];
+  double(*Fs)[NFD] = reinterpret_cast<double(*)[NFD]>(SF);
+  double *Ss = SF;
   const Grid &g = a.g;
   if (tile >= a.ntx * a.nty * a.ntz) return;
   // x fastest, then z, then y: the tiles in flight on an XCD share their z
@@ -854,10 +915,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       const int ii = win.i0 + w % WX, jj = win.j0 + (w / WX) % WY, kk = win.k0 + w / (WX * WY);
       ok[it] = w < NW && ii <= g.mx - 1 && jj <= g.my - 1 && kk <= g.mz - 1;
       if (ok[it]) {
-        const long long m = nid(g, kk, jj, ii);
+        const unsigned mo = node_off(g, kk, jj, ii);
 #pragma unroll
-        for (int c = 0; c < 4; c++) qv[it][c] = a.q[m + c * np];
-        qv[it][4] = a.p[m];
+        for (int c = 0; c < 4; c++) qv[it][c] = ldo(a.q + c * np, mo);
+        qv[it][4] = ldo(a.p, mo);
       }
     }
 #pragma unroll
@@ -879,12 +940,153 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     // faces of this node: lo at the node, hi one step along d
     const int lo = (lz * ey + ly) * ex + lx;
     const int hi = lo + (d == 0 ? 1 : d == 1 ? ex : ex * ey);
+    // -twoD 1 computes no xi face (rhs.c:1487-1714, 2646-2814, 3465-3706):
+    // their fluxes and LES stresses are the zeros of fresh Vecs
+    const bool skip_d = d == 0 && a.twoD == 1;
+    if constexpr (STG) {
+      // ---- staged form: thread t owns face t of direction d
+      constexpr int ns = NSM;
+      const int bx = TX + 4 * (d == 0), by = TY + 4 * (d == 1), bz = TZ + 4 * (d == 2);
+      const int ox = i0 - 2 * (d == 0), oy = j0 - 2 * (d == 1), oz = k0 - 2 * (d == 2);
+      const int bst = d == 0 ? 1 : d == 1 ? bx : bx * by;  // box slot step along d
+      const int fx = t % ex, fy = (t / ex) % ey, fz = t / (ex * ey);
+      const int fi = i0 + fx - (d == 0), fj = j0 + fy - (d == 1), fk = k0 + fz - (d == 2);
+      const bool fval = t < nfd && fi <= g.mx - 2 && fj <= g.my - 2 && fk <= g.mz - 2;
+      // ---- one load phase: the node box (the tile widened by two nodes
+      // along d: state, p and the metric row of the contravariant velocity),
+      // then the face's metrics and viscous inputs; all in flight together
+      constexpr int NBI = (NSM + TL::NT - 1) / TL::NT;
+      const int nb = bx * by * bz;
+      const double *mtd = metric(a, d);
+      double bv[NBI][9];
+      bool bok[NBI];
+#pragma unroll
+      for (int it = 0; it < NBI; it++) {
+        const int w = t + it * TL::NT;
+        const int ii = ox + w % bx, jj = oy + (w / bx) % by, kk = oz + w / (bx * by);
+        bok[it] = w < nb && ii >= 0 && jj >= 0 && kk >= 0 && ii <= g.mx - 1 && jj <= g.my - 1 && kk <= g.mz - 1;
+        if (bok[it]) {
+          const unsigned mo = node_off(g, kk, jj, ii);
+#pragma unroll
+          for (int c = 0; c < 5; c++) bv[it][c] = ldo(a.q + c * np, mo);
+          bv[it][5] = ldo(a.p, mo);
+#pragma unroll
+          for (int c = 0; c < 3; c++) bv[it][6 + c] = ldo(mtd + c * np, mo);
+        }
+      }
+      const bool fload = fval && !skip_d;
+      double fm[NFM][2];
+      double fin[4] = {0., 0., 0., 0.};  // LES: nu_t at A and B, filter width at A, rho at A + 1
+      if (fload) {
+        const unsigned oA = node_off(g, fk, fj, fi);
+        const unsigned oB = oA + 8u * (unsigned)(d == 0 ? 1 : d == 1 ? g.sy : g.sz);
+        if (VISC) {
+          const double *mt[3] = {a.csi, a.eta, a.zet};
+#pragma unroll
+          for (int c = 0; c < 9; c++) {
+            fm[c][0] = ldo(mt[c / 3] + (c % 3) * np, oA);
+            fm[c][1] = ldo(mt[c / 3] + (c % 3) * np, oB);
+          }
+          fm[9][0] = ldo(a.aj, oA);
+          fm[9][1] = ldo(a.aj, oB);
+          if (LES) {
+            fin[0] = ldo(a.nut, oA);
+            fin[1] = ldo(a.nut, oB);
+            fin[2] = ldo(a.fil2, oA);
+            if (!a.suth) fin[3] = ldo(a.q, oA + 8u);
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            fm[c][0] = ldo(mtd + c * np, oA);
+            fm[c][1] = ldo(mtd + c * np, oB);
+          }
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < NBI; it++) {
+        if (!bok[it]) continue;
+        const int w = t + it * TL::NT;
+#pragma unroll
+        for (int c = 0; c < 6; c++) Ss[c * ns + w] = bv[it][c];
+        Ss[6 * ns + w] = contra(bv[it][6], bv[it][7], bv[it][8], bv[it][1], bv[it][2], bv[it][3], bv[it][0]);
+      }
+      // the face's averages (9 metrics and aj, or the metric row of d), then
+      // its viscous coefficients: kept by the thread, which computes both
+      // fluxes of its face
+      double fa[NFM];
+      if (fload) {
+        constexpr int NAV = VISC ? 10 : 3;
+#pragma unroll
+        for (int c = 0; c < NAV; c++) fa[c] = 0.5 * (fm[c][0] + fm[c][1]);
+      }
+      __syncthreads();
+      // ---- convective flux of the face, and its viscous coefficients from
+      // the box's densities and pressures
+      double F[5] = {0., 0., 0., 0., 0.};
+      double cf[3];
+      if (fload) {
+        const Sv sv{Ss, ns, ((fk - oz) * by + (fj - oy)) * bx + (fi - ox), bst, true};
+        double h[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) h[c] = fa[VISC ? 3 * d + c : c];
+        conv_face_les<T, SCH, CEN>(a, sv, d, fk, fj, fi, F, h);
+        if (VISC) {
+          visc_coef<T, LES, CEN>(a, d, fk, fj, fi, fa[9], Ss[sv.o0], Ss[sv.o0 + bst], Ss[5 * ns + sv.o0],
+                            Ss[5 * ns + sv.o0 + bst], fin, cf);
+#pragma unroll
+          for (int q = 0; q < 3; q++) fa[9 + q] = cf[q];
+        }
+      }
+      // face densities of the LES energy term: rho at n - st, n, n + st along d
+      T rl = T(0), rh = T(0);
+      if (les_e) {
+        const int on = ((gk - oz) * by + (gj - oy)) * bx + (gi - ox);
+        const double r0 = Ss[on];
+        rl = T((r0 + Ss[on - bst]) * 0.5);
+        rh = T((Ss[on + bst] + r0) * 0.5);
+      }
+      __syncthreads();  // the box is read: the flux buffer takes its memory
+      if (fval) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) Fs[q][t] = F[q];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 5; q++) conv[q] = d == 0 ? Fs[q][hi] - Fs[q][lo] : conv[q] + Fs[q][hi] - Fs[q][lo];
+      __syncthreads();
+      if (VISC) {
+        // ---- viscous flux of the face from the window and its face data
+        if (fval) {
+          T G[4] = {T(0), T(0), T(0), T(0)}, al[3] = {T(0), T(0), T(0)};
+          if (!skip_d) visc_face_s<TL, T, LES>(a, Ws, win, d, fk, fj, fi, fa, 1, G, al);
+#pragma unroll
+          for (int q = 0; q < 4; q++) Fs[q][t] = G[q];
+          if (LES) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) Fs[4 + q][t] = al[q];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          visc[q + 1] = d == 0 ? Fs[q][hi] - Fs[q][lo] : visc[q + 1] + Fs[q][hi] - Fs[q][lo];
+        if (les_e) {
+          const T p0 = rh * T(Fs[4][hi]), p1 = rh * T(Fs[5][hi]), p2 = rh * T(Fs[6][hi]);
+          E0 = d == 0 ? p0 - rl * T(Fs[4][lo]) : E0 + p0 - rl * T(Fs[4][lo]);
+          E1 = d == 0 ? p1 - rl * T(Fs[5][lo]) : E1 + p1 - rl * T(Fs[5][lo]);
+          E2 = d == 0 ? p2 - rl * T(Fs[6][lo]) : E2 + p2 - rl * T(Fs[6][lo]);
+        }
+        __syncthreads();
+      }
+      continue;
+    }
     // Two-pass forms whose faces of every direction fit one pass of the
     // workgroup: the non-metric inputs of this thread's viscous face (f = t)
     // are loaded right after its convective face, and stay in flight through
     // the convective sums (barriers wait for LDS traffic only); loading the
     // metric part there too spills registers.
-    constexpr bool PF = VISC && !MERGE && TL::NFI <= TL::NT && TL::NFJ <= TL::NT && TL::NFK <= TL::NT;
+    constexpr bool PF = VISC && !MERGE && ONEPASS;
     VisIn vpre;
     // face densities of the LES energy term: rho at n, n - st, n + st along d
     double rle[3];
@@ -894,34 +1096,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       rle[1] = a.q[n - st];
       rle[2] = a.q[n + st];
     }
-    // ---- node data of direction d -> LDS: the state, p and (LES) the
-    // contravariant velocity, each computed once per node instead of once per
-    // stencil it belongs to
-    const int bx = TX + 4 * (d == 0), by = TY + 4 * (d == 1), bz = TZ + 4 * (d == 2);
-    const int ox = i0 - 2 * (d == 0), oy = j0 - 2 * (d == 1), oz = k0 - 2 * (d == 2);
-    constexpr int ns = NSM;
-    if (STG) {
-      const int nb = bx * by * bz;
-      const double *mt = metric(a, d);
-      for (int w = t; w < nb; w += TL::NT) {
-        const int ii = ox + w % bx, jj = oy + (w / bx) % by, kk = oz + w / (bx * by);
-        if (ii < 0 || jj < 0 || kk < 0 || ii > g.mx - 1 || jj > g.my - 1 || kk > g.mz - 1) continue;
-        const long long m = nid(g, kk, jj, ii);
-        const double rd = a.q[m], ru = a.q[m + np], rv = a.q[m + 2 * np], rw = a.q[m + 3 * np];
-        Ss[w] = rd;
-        Ss[ns + w] = ru;
-        Ss[2 * ns + w] = rv;
-        Ss[3 * ns + w] = rw;
-        Ss[4 * ns + w] = a.q[m + 4 * np];
-        Ss[5 * ns + w] = a.p[m];
-        // the contravariant velocity of the four-point flux (conv_face_les)
-        Ss[6 * ns + w] = contra(mt[m], mt[m + np], mt[m + 2 * np], ru, rv, rw, rd);
-      }
-      __syncthreads();
-    }
-    // -twoD 1 computes no xi face (rhs.c:1487-1714, 2646-2814, 3465-3706):
-    // their fluxes and LES stresses are the zeros of fresh Vecs
-    const bool skip_d = d == 0 && a.twoD == 1;
     // ---- convective fluxes of direction d -> LDS
     for (int f = t; f < nfd; f += TL::NT) {
       const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
@@ -932,15 +1106,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         for (int q = 0; q < FC; q++) Fs[q][f] = 0.0;
         continue;
       }
-      Sv sv{nullptr, ns, 0, 0, false};
-      if (STG) {
-        sv.on = true;
-        sv.S = Ss;
-        sv.o0 = ((fk - oz) * by + (fj - oy)) * bx + (fi - ox);
-        sv.st = d == 0 ? 1 : d == 1 ? bx : bx * by;
-      }
+      const Sv sv{nullptr, 0, 0, 0, false};
       double F[5];
-      if (LES) conv_face_les<T, SCH>(a, sv, d, fk, fj, fi, F);
+      if (LES) conv_face_les<T, SCH>(a, sv, d, fk, fj, fi, F, nullptr);
       else conv_face_plain<T, SCH>(a, sv, d, fk, fj, fi, F);
 #pragma unroll
       for (int q = 0; q < 5; q++) Fs[q][f] = F[q];
@@ -1019,13 +1187,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   }
   // the node's update inputs
   double ajn = 0., qin[5] = {0., 0., 0., 0., 0.}, qol[5] = {0., 0., 0., 0., 0.};
+  const unsigned no = (unsigned)n * 8u;
   if (valid) {
-    ajn = a.aj[n];
+    ajn = ldo(a.aj, no);
 #pragma unroll
-    for (int q = 0; q < 5; q++) qin[q] = a.q[n + q * np];
+    for (int q = 0; q < 5; q++) qin[q] = ldo(a.q + q * np, no);
     if (MODE != 0) {
 #pragma unroll
-      for (int q = 0; q < 5; q++) qol[q] = a.qold[n + q * np];
+      for (int q = 0; q < 5; q++) qol[q] = ldo(a.qold + q * np, no);
     }
   }
   if (LES && a.ratio != 0.0 && valid) {
@@ -1080,17 +1249,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     for (int q = 0; q < 5; q++) qn[q] = a.a_old * qol[q] + a.a_rhs * rhs[q] + a.a_cur * qin[q];
   }
 #pragma unroll
-  for (int q = 0; q < 5; q++) a.qo[n + q * np] = qn[q];
+  for (int q = 0; q < 5; q++) sto(a.qo + q * np, no, qn[q]);
   // EqOfState (rhs.c:4853-4857)
   a.po[n] = (a.gamma - 1) * (qn[4] - 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) * rcp64(qn[0]));
 }
 
-template <class TL, typename T, bool LES, bool VISC, int MODE, int SCH = 0>
+template <class TL, typename T, bool LES, bool VISC, int MODE, int SCH = 0, bool CEN = false>
 __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWaves))) void stage_kernel(StageArgs a) {
   // XCD-aware tile numbering: blockIdx % 8 selects the XCD, each XCD walks a
   // contiguous range of tiles and keeps their shared halos in its own L2.
   const int b = blockIdx.x;
-  stage_tile<TL, T, LES, VISC, MODE, SCH>(a, (b & 7) * a.per_xcd + (b >> 3), b);
+  stage_tile<TL, T, LES, VISC, MODE, SCH, CEN>(a, (b & 7) * a.per_xcd + (b >> 3), b);
 }
 
 // Batched stage: the workgroups of one launch cover the tiles of several
@@ -1203,13 +1372,22 @@ void tile_counts(const Grid &g, StageArgs &a) {
   a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
 }
 
-template <class TL, typename T, bool LES, bool VISC, int SCH = 0>
+template <class TL, typename T, bool LES, bool VISC, int SCH = 0, bool CEN = false>
 hipError_t launch_t(const StageArgs &a, int mode, int nblocks, hipStream_t s) {
   if (mode == 0)
-    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 0, SCH>), dim3(nblocks), dim3(TL::NT), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 0, SCH, CEN>), dim3(nblocks), dim3(TL::NT), 0, s, a);
   else
-    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 1, SCH>), dim3(nblocks), dim3(TL::NT), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<TL, T, LES, VISC, 1, SCH, CEN>), dim3(nblocks), dim3(TL::NT), 0, s, a);
   return hipGetLastError();
+}
+
+// sigma = 1 on every face: no shock sensor field and s_x = s_y = s_z = 1
+// inside and on the boundary
+bool sigma_is_one(const StageArgs &a) {
+  if (a.sgm) return false;
+  for (int d = 0; d < 3; d++)
+    if (a.sig.in[d] != 1.0 || a.sig.bnd[d] != 1.0) return false;
+  return true;
 }
 
 // the build-only schemes (weno5 / roe, FP64 only) in the default 8x4x3 tile
@@ -1228,6 +1406,16 @@ hipError_t launch_shape(const cles_params &p, StageArgs &a, int mode, hipStream_
   tile_counts<TL>(a.g, a);
   const int nblocks = a.per_xcd * 8;
   const bool les = p.les != 0, visc = !p.invicid;
+  if constexpr (std::is_same<TL, TileH>::value) {
+    // the default tile of the LES forms: central flux only where sigma = 1
+    if (les && sigma_is_one(a)) {
+      if (p.precision == CLES_PREC_FP32BULK)
+        return visc ? launch_t<TL, float, true, true, 0, true>(a, mode, nblocks, s)
+                    : launch_t<TL, float, true, false, 0, true>(a, mode, nblocks, s);
+      return visc ? launch_t<TL, double, true, true, 0, true>(a, mode, nblocks, s)
+                  : launch_t<TL, double, true, false, 0, true>(a, mode, nblocks, s);
+    }
+  }
   if (p.precision == CLES_PREC_FP32BULK) {
     if (les) return visc ? launch_t<TL, float, true, true>(a, mode, nblocks, s) : launch_t<TL, float, true, false>(a, mode, nblocks, s);
     return visc ? launch_t<TL, float, false, true>(a, mode, nblocks, s) : launch_t<TL, float, false, false>(a, mode, nblocks, s);
