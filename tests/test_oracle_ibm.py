@@ -384,3 +384,48 @@ def test_forces_of_hydrostatic_pressure_count_the_body():
     assert abs(out[0]) < 1e-12 and abs(out[1]) < 1e-12
     # only the pressure part acts: the fluid is at rest
     assert np.array_equal(out[0:3], out[3:6])
+
+
+def test_viscous_forces_of_a_shear_flow_over_a_box():
+    """Viscous part of Calc_forces_SI2 by a closed form.  A box body on a
+    uniform grid (h = 1/8, binary exact), uniform pressure (no pressure force),
+    and u = g |z - zc| with zc the box's mid-plane: every velocity difference
+    of the stress stencils (central or one-sided) lies on one side of the kink
+    at the nodes with a fluid face above or below, so there du/dz = +-g exactly
+    and the stress is T_zx = (1/ren) g sign(z - zc).  A k+ face adds T_zx h^2
+    to F_x and a k- face subtracts it: nodes with fluid above and below (the
+    sides) cancel, the top and bottom layers add, F_x = (1/ren) g h^2
+    (N_top + N_bottom) with the layers counted from nvert alone; F_z from the
+    i+- faces cancels between the box's two sides, F_y is zero.  A wrong
+    stress factor, face sign or face selection fails."""
+    n, L = 16, 2.0
+    coor, h = cases.cartesian_grid(n, L=L)
+    z = coor[..., 2]
+    zc, g = 1.0, 0.5
+    Q = cases.conserved(np.ones(z.shape), g * np.abs(z - zc), 0 * z, 0 * z, 5.0 + 0 * z)
+    cfg = dict(mx=n + 2, my=n + 2, mz=n + 2, bctype=(7,) * 6, invicid=0, les=0, isothermal=2, ren=64.0,
+               Ma=0.3, dt=1e-3)
+    o = po.Oracle(cfg, coor)
+    o.set_state(Q)
+    V, T = cases.box_surface((0.53, 0.61, zc - 0.43), (1.41, 1.37, zc + 0.43))
+    assert o.ibm_search(V, T) > 0
+    o.form_rhs(0)  # CalcUCat: the forces read the velocity of the last one
+    out = o.ibm_forces(center=(1.0, 1.0, 1.0))
+    nv = o.nvert()
+    o.close()
+    fluid = nv < 0.9
+    ib = (nv > 0.9) & (nv < 2.5)
+    kk = np.arange(n + 2)[:, None, None] * np.ones((1, n + 2, n + 2), int)
+    inner = ib & (kk >= 1) & (kk <= n)
+    up = np.zeros_like(fluid)
+    dn = np.zeros_like(fluid)
+    up[:-1] = fluid[1:]
+    dn[1:] = fluid[:-1]
+    s = up.astype(int) - dn.astype(int)                  # k+ face minus k- face
+    sign = np.sign(z - zc)
+    assert np.all(sign[inner & (s != 0)] == s[inner & (s != 0)])  # top layer above zc, bottom below
+    nlayer = int(np.abs(s[inner]).sum())
+    expect = (1.0 / cfg["ren"]) * g * h * h * nlayer
+    assert nlayer > 0 and abs(out[0] - expect) <= 1e-13 * expect, (out[0], expect, nlayer)
+    assert abs(out[2]) <= 1e-13 * expect and out[1] == 0.0
+    assert np.all(out[3:6] == 0.0)                       # uniform pressure: no pressure force

@@ -614,3 +614,50 @@ def test_homogeneous_j_alone_is_pointwise():
     inner = (slice(1, -1),) * 3
     want = _clip(np.maximum(lm[inner] / (mm[inner] + 1e-6), 1e-4))
     assert np.allclose(cs[inner], want, rtol=1e-12)
+
+
+# -------------------------------------------- Do_averaging (les.c:1664-1747)
+def test_averaging_sums_the_states_and_the_last_calcucat_velocity():
+    """Do_averaging after each of two steps: Q_sum and P_sum are the sums of
+    the states after the steps (P from the equation of state), u^2 and the
+    cross products (uv, vw, wu) the sums over the velocity of each step's last
+    CalcUCat, i.e. of the input of its last RK stage, composed here from single
+    residual evaluations as in the Runge-Kutta pin.  Summing the velocity of
+    the state after the step instead, or swapping a cross product, fails."""
+    cfg, coor, Q0 = cases.vortex_case(N=8)
+    cfg.update(invicid=0, ren=200.0, fluxorder=3)
+    dt = cfg["dt"]
+    g = cfg.get("gamma", GAMMA)
+    Q = Q0
+    qs = np.zeros_like(Q0)
+    ps = np.zeros(Q0.shape[:3])
+    u2s = np.zeros(Q0.shape[:3] + (3,))
+    ucs = np.zeros(Q0.shape[:3] + (3,))
+    ucs_end = np.zeros(Q0.shape[:3] + (3,))
+    for _ in range(2):
+        S = Q
+        inputs = []
+        for s, (b, a) in enumerate([(1.0, 0.0), (0.75, 0.25), (1 / 3, 2 / 3)]):
+            inputs.append(S)
+            L = _rhs_of(cfg, coor, S)
+            Sn = Q + dt * L if s == 0 else b * Q + a * (S + dt * L)
+            S = _seams_from(Sn, S)
+        Q = S
+        u = inputs[-1][..., 1:4] / inputs[-1][..., 0:1]
+        qs += Q
+        ps += (g - 1) * (Q[..., 4] - 0.5 * (Q[..., 1] ** 2 + Q[..., 2] ** 2 + Q[..., 3] ** 2) / Q[..., 0])
+        u2s += u * u
+        ucs += np.stack([u[..., 0] * u[..., 1], u[..., 1] * u[..., 2], u[..., 2] * u[..., 0]], axis=-1)
+        ue = Q[..., 1:4] / Q[..., 0:1]
+        ucs_end += np.stack([ue[..., 0] * ue[..., 1], ue[..., 1] * ue[..., 2], ue[..., 2] * ue[..., 0]], axis=-1)
+    o = pyoracle.Oracle(cfg, coor)
+    o.set_state(Q0)
+    o.step(2, averaging=True)
+    q, p, u2, uc = o.averages()
+    o.close()
+    _close(q, qs, 1e-13)
+    _close(p, ps, 1e-12)
+    _close(u2, u2s, 1e-12)
+    _close(uc, ucs, 1e-12)
+    assert np.abs(uc - ucs_end).max() > 1e-6 * np.abs(ucs).max()
+    assert np.abs(uc - ucs[..., [1, 2, 0]]).max() > 1e-3 * np.abs(ucs).max()
