@@ -48,9 +48,9 @@
 // launch counts of one batched step
 struct BatchPlan {
   // launch mode: members per group, workgroups per stage and group, seam blocks per stage
-  int ngroup[8] = {}, nblk[3][8] = {}, nbnd[3] = {};
+  int ngroup[16] = {}, nblk[3][16] = {}, nbnd[3] = {};
   // persistent mode: phases per group
-  int nph[8] = {};
+  int nph[16] = {};
 };
 
 struct cles_batch {
@@ -89,7 +89,9 @@ struct cles_batch {
 namespace cles {
 namespace {
 
-constexpr int kGroups = 8;  // (fp32, LES, viscous)
+// (fp32, LES, viscous, sigma = 1 on every face of an LES member: the central
+// instance of the stage kernel, in launch mode)
+constexpr int kGroups = 16;
 constexpr int kQueues = 8;
 
 int batch_error(cles_batch *b, int code, const char *msg) {
@@ -132,8 +134,15 @@ unsigned *abort_flag(cles_batch *b) { return b->d_ctr + ((size_t)kGroups * kQueu
 bool les_on(const cles_ctx *c) { return c->p.les != 0; }
 // flow_step runs the model before the stages (solvers.c:16-30)
 bool les_model(const cles_ctx *c) { return c->p.les && !c->p.invicid && c->p.les != 10; }
+bool sigma_one(const cles_ctx *c) {
+  if (c->sgm) return false;
+  for (int d = 0; d < 3; d++)
+    if (c->sig.in[d] != 1.0 || c->sig.bnd[d] != 1.0) return false;
+  return true;
+}
 int group_of(const cles_ctx *c) {
-  return (c->p.precision == CLES_PREC_FP32BULK ? 4 : 0) + (les_on(c) ? 2 : 0) + (c->p.invicid ? 0 : 1);
+  return (c->p.precision == CLES_PREC_FP32BULK ? 4 : 0) + (les_on(c) ? 2 : 0) + (c->p.invicid ? 0 : 1) +
+         (les_on(c) && sigma_one(c) ? 8 : 0);
 }
 int member_tile(const cles_batch *b, const cles_ctx *c) { return les_on(c) ? 7 : b->tile; }
 int last_input(const cles_batch *b, const cles_ctx *c) { return (c->cur + (b->nstages == 3 ? 2 : 1)) % 3; }
@@ -359,7 +368,7 @@ hipError_t record_step(cles_batch *b, const BatchPlan &pl) {
       a.done = b->d_ctr + (size_t)kGroups * kQueues * kCtrStride;
       a.abort = abort_flag(b);
       a.spin_limit = b->spin_limit;
-      e = launch_stage_persist(b->tile, gi >= 4, gi & 1, (gi >> 1) & 1, a, b->stream);
+      e = launch_stage_persist(b->tile, (gi >> 2) & 1, gi & 1, (gi >> 1) & 1, a, b->stream);
     }
   } else {
     const size_t sa = stage_args_bytes(), ba = seam_args_bytes();
@@ -367,7 +376,8 @@ hipError_t record_step(cles_batch *b, const BatchPlan &pl) {
       for (int gi = 0; gi < kGroups && e == hipSuccess; gi++) {
         if (!pl.ngroup[gi]) continue;
         const size_t off = (size_t)s * kGroups + gi;
-        e = launch_stage_batch(b->tile, gi >= 4, gi & 1, (gi >> 1) & 1, (const char *)b->d_stage + off * b->cap * sa,
+        e = launch_stage_batch(b->tile, (gi >> 2) & 1, gi & 1, (gi >> 1) & 1, gi >> 3,
+                               (const char *)b->d_stage + off * b->cap * sa,
                                b->d_tstart + off * (b->cap + 1), pl.ngroup[gi], pl.nblk[s][gi], b->stream);
       }
       if (e == hipSuccess)

@@ -310,7 +310,9 @@ int get_coor(cles_ctx *c, double *xyz);
 // batched stages of several periodic grids (serving)
 size_t stage_args_bytes();
 int stage_batch_args(cles_ctx *c, int in, int out, int istage, const double coef[3], int tile, void *dst);
-hipError_t launch_stage_batch(int tile, int fp32, int visc, int les, const void *tab, const int *tstart, int nreq, int nblocks,
+// cen: every member has sigma = 1 on every face (the central instance of the LES forms)
+hipError_t launch_stage_batch(int tile, int fp32, int visc, int les, int cen, const void *tab, const int *tstart, int nreq,
+                              int nblocks,
                               hipStream_t s);
 size_t seam_args_bytes();
 int seam_batch_args(cles_ctx *c, int in, int out, void *dst);

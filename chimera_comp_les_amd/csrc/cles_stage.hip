@@ -1265,7 +1265,7 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWa
 // Batched stage: the workgroups of one launch cover the tiles of several
 // independent grids; tstart[r] is the first workgroup of grid r (prefix sum
 // of the workgroup counts, 8 per_xcd each, tstart[nreq] = total).
-template <class TL, typename T, bool LES, bool VISC, int MODE>
+template <class TL, typename T, bool LES, bool VISC, int MODE, bool CEN = false>
 __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWaves))) void stage_batch_kernel(const StageArgs *__restrict__ tab, const int *__restrict__ tstart,
                                                          int nreq) {
   const int b = blockIdx.x;
@@ -1276,7 +1276,7 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(kStageWa
   }
   // the grid's own XCD-aware numbering (every grid starts on a multiple of 8)
   const int lb = b - tstart[lo];
-  stage_tile<TL, T, LES, VISC, MODE>(tab[lo], (lb & 7) * tab[lo].per_xcd + (lb >> 3), b);
+  stage_tile<TL, T, LES, VISC, MODE, 0, CEN>(tab[lo], (lb & 7) * tab[lo].per_xcd + (lb >> 3), b);
 }
 
 // Persistent batched stages (serving): the workgroups pull items from eight
@@ -1424,16 +1424,23 @@ hipError_t launch_shape(const cles_params &p, StageArgs &a, int mode, hipStream_
   return visc ? launch_t<TL, double, false, true>(a, mode, nblocks, s) : launch_t<TL, double, false, false>(a, mode, nblocks, s);
 }
 
-template <class TL, typename T, bool LES, bool VISC>
+template <class TL, typename T, bool LES, bool VISC, bool CEN = false>
 hipError_t launch_batch_t(const StageArgs *tab, const int *tstart, int nreq, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((stage_batch_kernel<TL, T, LES, VISC, 1>), dim3(nblocks), dim3(TL::NT), 0, s, tab, tstart, nreq);
+  hipLaunchKernelGGL((stage_batch_kernel<TL, T, LES, VISC, 1, CEN>), dim3(nblocks), dim3(TL::NT), 0, s, tab, tstart,
+                     nreq);
   return hipGetLastError();
 }
 
 template <class TL>
-hipError_t launch_batch_shape(int fp32, int visc, int les, const StageArgs *tab, const int *tstart, int nreq,
+hipError_t launch_batch_shape(int fp32, int visc, int les, int cen, const StageArgs *tab, const int *tstart, int nreq,
                               int nblocks, hipStream_t s) {
   if (les) {  // the LES forms use the 8x4x3 tile
+    if (cen) {
+      if (fp32) return visc ? launch_batch_t<TileH, float, true, true, true>(tab, tstart, nreq, nblocks, s)
+                            : launch_batch_t<TileH, float, true, false, true>(tab, tstart, nreq, nblocks, s);
+      return visc ? launch_batch_t<TileH, double, true, true, true>(tab, tstart, nreq, nblocks, s)
+                  : launch_batch_t<TileH, double, true, false, true>(tab, tstart, nreq, nblocks, s);
+    }
     if (fp32) return visc ? launch_batch_t<TileH, float, true, true>(tab, tstart, nreq, nblocks, s)
                           : launch_batch_t<TileH, float, true, false>(tab, tstart, nreq, nblocks, s);
     return visc ? launch_batch_t<TileH, double, true, true>(tab, tstart, nreq, nblocks, s)
@@ -1579,18 +1586,18 @@ int stage_tiles(cles_ctx *c, int tile) {
 #undef CALL
 }
 
-hipError_t launch_stage_batch(int tile, int fp32, int visc, int les, const void *tab, const int *tstart, int nreq,
-                              int nblocks, hipStream_t s) {
+hipError_t launch_stage_batch(int tile, int fp32, int visc, int les, int cen, const void *tab, const int *tstart,
+                              int nreq, int nblocks, hipStream_t s) {
   const StageArgs *t = (const StageArgs *)tab;
   switch (tile) {
-    case 1: return launch_batch_shape<TileB>(fp32, visc, les, t, tstart, nreq, nblocks, s);
-    case 2: return launch_batch_shape<TileC>(fp32, visc, les, t, tstart, nreq, nblocks, s);
-    case 3: return launch_batch_shape<TileD>(fp32, visc, les, t, tstart, nreq, nblocks, s);
-    case 4: return launch_batch_shape<TileE>(fp32, visc, les, t, tstart, nreq, nblocks, s);
-    case 5: return launch_batch_shape<TileF>(fp32, visc, les, t, tstart, nreq, nblocks, s);
-    case 6: return launch_batch_shape<TileG>(fp32, visc, les, t, tstart, nreq, nblocks, s);
-    case 7: return launch_batch_shape<TileH>(fp32, visc, les, t, tstart, nreq, nblocks, s);
-    default: return launch_batch_shape<TileA>(fp32, visc, les, t, tstart, nreq, nblocks, s);
+    case 1: return launch_batch_shape<TileB>(fp32, visc, les, cen, t, tstart, nreq, nblocks, s);
+    case 2: return launch_batch_shape<TileC>(fp32, visc, les, cen, t, tstart, nreq, nblocks, s);
+    case 3: return launch_batch_shape<TileD>(fp32, visc, les, cen, t, tstart, nreq, nblocks, s);
+    case 4: return launch_batch_shape<TileE>(fp32, visc, les, cen, t, tstart, nreq, nblocks, s);
+    case 5: return launch_batch_shape<TileF>(fp32, visc, les, cen, t, tstart, nreq, nblocks, s);
+    case 6: return launch_batch_shape<TileG>(fp32, visc, les, cen, t, tstart, nreq, nblocks, s);
+    case 7: return launch_batch_shape<TileH>(fp32, visc, les, cen, t, tstart, nreq, nblocks, s);
+    default: return launch_batch_shape<TileA>(fp32, visc, les, cen, t, tstart, nreq, nblocks, s);
   }
 }
 
