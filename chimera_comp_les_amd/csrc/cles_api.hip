@@ -118,6 +118,9 @@ int free_buffer(const cles_ctx *c, int a, int b) {
 
 int check_params(const cles_params *p) {
   if (p->mx < 5 || p->my < 5 || p->mz < 5) return set_error(nullptr, CLES_EINVAL, "grid must have at least 5 nodes per direction");
+  // the stage kernel addresses a plane with 32-bit byte offsets
+  if ((long long)p->mx * p->my * p->mz > (1LL << 29) - 64)
+    return set_error(nullptr, CLES_EINVAL, "grid has more than 2^29 - 64 nodes (a plane must stay below 4 GB)");
   for (int d = 0; d < 3; d++)
     if ((p->bctype[2 * d] == 7) != (p->bctype[2 * d + 1] == 7))
       return set_error(nullptr, CLES_EINVAL, "periodic direction %d needs bctype 7 on both sides", d);

@@ -74,3 +74,9 @@ def test_create_rejects_unsupported_options_before_device_use():
     assert e == -1 and "twoD" in msg
     e, msg = _create_err(weno5=1, precision=1)
     assert e == -1 and "FP64" in msg
+
+
+def test_create_rejects_grids_beyond_32_bit_plane_offsets():
+    # a plane of more than 2^29 nodes would not fit the stage kernel's 32-bit byte offsets
+    e, msg = _create_err(mx=1024, my=1024, mz=1024)
+    assert e == -1 and "2^29" in msg
