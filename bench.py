@@ -22,6 +22,7 @@ one thread on a bounded sample).
     python bench.py                                    # TGV-LES 256^3 FP64, WENO3/RK3
     python bench.py --case vortex                      # isentropic vortex 256^3, WENO3/Rusanov RK3
     python bench.py --scheme muscl                     # TGV-LES 256^3, MUSCL/RK2
+    python bench.py --case vortex --scheme weno5roe    # isentropic vortex 256^3, WENO5 + Roe flux, RK3
     python bench.py --precision fp32bulk               # FP32 fluxes, FP64 metrics/RK/EOS
 """
 import argparse
@@ -64,7 +65,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--N", type=int, default=256)
-    ap.add_argument("--scheme", choices=["weno", "muscl"], default="weno")
+    ap.add_argument("--scheme", choices=["weno", "muscl", "weno5roe"], default="weno",
+                    help="weno: the reference's WENO3 (RK3); muscl: MUSCL-minmod (RK2); weno5roe: WENO5 with the "
+                         "Roe flux (RK3, FP64), the schemes of the vortex configuration")
     ap.add_argument("--precision", choices=["fp64", "fp32bulk"], default="fp64")
     ap.add_argument("--case", choices=["tgv", "vortex", "sphere"], default="tgv")
     ap.add_argument("--cpu-N", type=int, default=None, help="CPU sample size (default: the GPU size)")
@@ -118,13 +121,15 @@ def oracle_rate(pyoracle, workload, args, N, threads, steps, mesh_case=False):
     from chimera_comp_les_amd import cases
     mesh = None
     if workload == "tgv":
-        cfg, coor, Q = cases.tgv_case(N=N, fluxorder=3 if args.scheme == "weno" else 2)
+        cfg, coor, Q = cases.tgv_case(N=N, fluxorder=2 if args.scheme == "muscl" else 3)
     elif workload == "sphere":
         cfg, coor, Q, V, T = cases.sphere_channel_case()
         mesh = (V, T)
     else:
         cfg, coor, Q = cases.vortex_case(N=N)
-        cfg = dict(cfg, fluxorder=3 if args.scheme == "weno" else 2)
+        cfg = dict(cfg, fluxorder=2 if args.scheme == "muscl" else 3)
+    if args.scheme == "weno5roe":
+        cfg = dict(cfg, weno5=1, roe=1)
     o = pyoracle.Oracle(cfg, coor, threads=threads)
     o.set_state(Q)
     del Q, coor
@@ -213,7 +218,11 @@ def main():
         return store_traffic(args, tkey)
     import chimera_comp_les_amd as cl
     from chimera_comp_les_amd import cases
-    fluxorder = 3 if args.scheme == "weno" else 2
+    fluxorder = 2 if args.scheme == "muscl" else 3
+    schemes = dict(weno5=1, roe=1) if args.scheme == "weno5roe" else {}
+    if schemes and args.precision != "fp64":
+        print("the WENO5 and Roe schemes run in the FP64 form only", file=sys.stderr)
+        return 2
     if args.case == "tgv":
         cfg, coor, Q = cases.tgv_case(N=args.N, fluxorder=fluxorder)
         # with the LES form's sigma = 1 the hybrid flux is the central flux
@@ -230,9 +239,10 @@ def main():
     else:
         cfg, coor, Q = cases.vortex_case(N=args.N)
         cfg = dict(cfg, fluxorder=fluxorder)
-        workload = "isentropic-vortex %d^3 curvilinear inviscid %s/Rusanov SSP-RK%d" % (
-            args.N, "WENO3" if fluxorder == 3 else "MUSCL", fluxorder)
+        flux = {"weno": "WENO3/Rusanov", "muscl": "MUSCL/Rusanov", "weno5roe": "WENO5 (Jiang-Shu)/Roe"}[args.scheme]
+        workload = "isentropic-vortex %d^3 curvilinear inviscid %s SSP-RK%d" % (args.N, flux, fluxorder)
         b_stage = B_STAGE_INVISCID
+    cfg = dict(cfg, **schemes)
     prec = cl.PREC_FP64 if args.precision == "fp64" else cl.PREC_FP32BULK
     s = cl.Solver(cfg, coor, precision=prec)
     s.set_state(Q)
