@@ -186,24 +186,32 @@ __device__ __forceinline__ long long seam_image(const Grid &g, int k, int j, int
 // the 27-point weighted sum of a quantity is 0.5 P(k-1) + P(k) + 0.5 P(k+1)
 // with P(k) the 9-point weighted sum over plane k; the weights sum to 8, so
 // vsum / wsum is vsum * 0.125.  A workgroup owns a 16 x 16 column of
-// interior nodes and walks a chunk of planes: each plane's node data (rho,
-// rhoU, rhoV, rhoW, the velocity lUcat holds after the gradient loop, the
-// six strain-rate components and |S| as the filter sees them: periodic face
-// copies of lSx..lSz, les.c:316-377, zero on other boundary nodes, |S|
-// without a periodic copy) with a one-node halo is staged in LDS once, every
-// thread forms the 22 plane sums of its node from it, and keeps the sums of
-// the two planes before in registers.
+// interior nodes and walks a chunk of planes: each plane's node data with a
+// one-node halo is staged in LDS once, every thread forms the plane sums of
+// its node from it, and keeps the sums of the two planes before in
+// registers.  The contraction runs as two launches that share no data, each
+// holding about half the sums, so that each fits 128 VGPRs (4 waves per SIMD;
+// one launch with all 22 sums needs 221 and runs at 2):
+//   PART 0, L_ij S_ij: sums of rho, rhoU, rhoV, rhoW and the six rho u_a u_b
+//     products (staged: rho, rhoU, rhoV, rhoW and the velocity lUcat holds
+//     after the gradient loop); writes LM;
+//   PART 1, M_ij S_ij: sums of rho, the six strain-rate components as the
+//     filter sees them (periodic face copies of lSx..lSz, les.c:316-377, zero
+//     on other boundary nodes) and their products with rho |S| (|S| without a
+//     periodic copy); writes MM.
+// Each sum is formed with the same operations, in the same order, as one
+// launch would form it.
 constexpr int FTX = 16, FTY = 16;
 constexpr int FBX = FTX + 2, FBY = FTY + 2, FNB = FBX * FBY;
 constexpr int FKC = 32;  // planes per workgroup
-constexpr int FNQ = 14;  // staged values per node
-constexpr int FNS = 22;  // plane sums per node
 constexpr double FILTER_INV_WSUM = 0.125;
 static_assert(FTX * FTY == 256, "one column per thread");
 
-template <typename T>
-__global__ __launch_bounds__(256) void les_filter_kernel(LesArgs<T> a, int ntx, int nty, int ntiles, int per_xcd) {
-  __shared__ T B[FNQ][FNB];
+template <typename T, int PART>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void les_filter_kernel(LesArgs<T> a, int ntx, int nty, int ntiles, int per_xcd) {
+  constexpr int NQ = PART == 0 ? 7 : 8;    // staged values per node
+  constexpr int NS = PART == 0 ? 10 : 13;  // plane sums per node
+  __shared__ T B[NQ][FNB];
   const Grid &g = a.g;
   const long long np = g.np;
   const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);  // contiguous tiles per XCD
@@ -217,7 +225,7 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs<T> a, int ntx, 
   const int i = i0 + lx, j = j0 + ly;
   const bool col = i <= g.mx - 2 && j <= g.my - 2;
   const int c0 = (ly + 1) * FBX + (lx + 1);  // slot of the column's node
-  T Pm[FNS], P0[FNS], Sx0[6];
+  T Pm[NS], P0[NS], Sx0[6];
   bool act0 = false;
   for (int kk = k0 - 1; kk <= k1 + 1; kk++) {
     // ---- stage plane kk
@@ -225,36 +233,49 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs<T> a, int ntx, 
       const int ii = i0 - 1 + w % FBX, jj = j0 - 1 + w / FBX;
       if (ii > g.mx - 1 || jj > g.my - 1) continue;
       const long long m = nid(g, kk, jj, ii);
-      const double r = a.q[m], ru = a.q[m + np], rv = a.q[m + 2 * np], rw = a.q[m + 3 * np];
+      const double r = a.q[m];
       B[0][w] = T(r);
-      B[1][w] = T(ru);
-      B[2][w] = T(rv);
-      B[3][w] = T(rw);
-      T u[3];
-      if (in_loop(a, kk, jj, ii)) {
-        const double ir = rcp64(r);
-        u[0] = T(ru * ir);
-        u[1] = T(rv * ir);
-        u[2] = T(rw * ir);
+      if (PART == 0) {
+        const double ru = a.q[m + np], rv = a.q[m + 2 * np], rw = a.q[m + 3 * np];
+        B[1][w] = T(ru);
+        B[2][w] = T(rv);
+        B[3][w] = T(rw);
+        T u[3];
+        if (in_loop(a, kk, jj, ii)) {
+          const double ir = rcp64(r);
+          u[0] = T(ru * ir);
+          u[1] = T(rv * ir);
+          u[2] = T(rw * ir);
+        } else {
+          u_stale(a, m, u);
+        }
+        B[4][w] = u[0];
+        B[5][w] = u[1];
+        B[6][w] = u[2];
       } else {
-        u_stale(a, m, u);
-      }
-      B[4][w] = u[0];
-      B[5][w] = u[1];
-      B[6][w] = u[2];
-      long long ms = m;
-      T S = T(0);
-      if (interior(g, kk, jj, ii)) S = a.grad[m + 6 * np];
-      else ms = seam_image(g, kk, jj, ii);
+        long long ms = m;
+        T S = T(0);
+        if (interior(g, kk, jj, ii)) S = a.grad[m + 6 * np];
+        else ms = seam_image(g, kk, jj, ii);
 #pragma unroll
-      for (int c = 0; c < 6; c++) B[7 + c][w] = ms >= 0 ? a.grad[ms + c * np] : T(0);
-      B[13][w] = S;
+        for (int c = 0; c < 6; c++) B[1 + c][w] = ms >= 0 ? a.grad[ms + c * np] : T(0);
+        B[7][w] = S;
+      }
+    }
+    const bool act1 = col && kk >= 1 && kk <= g.mz - 2 && !(nvat(a, nid(g, kk, j, i)) > 1.1);
+    // the node's own strain rates (it is interior when active): read here,
+    // used when the node's sums are complete, one plane later
+    T Sx1[6];
+    if (PART == 0) {
+      const long long n1 = act1 ? nid(g, kk, j, i) : 0;
+#pragma unroll
+      for (int c = 0; c < 6; c++) Sx1[c] = act1 ? a.grad[n1 + c * np] : T(0);
     }
     __syncthreads();
-    // ---- the 22 plane sums of this column's node
-    T P1[FNS];
+    // ---- the plane sums of this column's node
+    T P1[NS];
 #pragma unroll
-    for (int q = 0; q < FNS; q++) P1[q] = T(0);
+    for (int q = 0; q < NS; q++) P1[q] = T(0);
     // one row of the 3x3 neighbourhood at a time: the LDS reads of a row are
     // in flight together, not those of all nine nodes (registers)
 #pragma unroll 1
@@ -263,85 +284,100 @@ __global__ __launch_bounds__(256) void les_filter_kernel(LesArgs<T> a, int ntx, 
       for (int Pd = 0; Pd < 3; Pd++) {
         const int o = c0 + (Qd - 1) * FBX + (Pd - 1);
         const T w = T((Qd == 1 ? 1.0 : 0.5) * (Pd == 1 ? 1.0 : 0.5));
-        const T rho = B[0][o], ru = B[1][o], rv = B[2][o], rw = B[3][o];
-        const T u0 = B[4][o], u1 = B[5][o], u2 = B[6][o];
+        const T rho = B[0][o];
         P1[0] = fma(w, rho, P1[0]);
-        P1[1] = fma(w, ru, P1[1]);
-        P1[2] = fma(w, rv, P1[2]);
-        P1[3] = fma(w, rw, P1[3]);
-        P1[4] = fma(w, ru * u0, P1[4]);
-        P1[5] = fma(w, ru * u1, P1[5]);
-        P1[6] = fma(w, ru * u2, P1[6]);
-        P1[7] = fma(w, rv * u1, P1[7]);
-        P1[8] = fma(w, rv * u2, P1[8]);
-        P1[9] = fma(w, rw * u2, P1[9]);
-        const T rS = rho * B[13][o];
+        if (PART == 0) {
+          const T ru = B[1][o], rv = B[2][o], rw = B[3][o];
+          const T u0 = B[4][o], u1 = B[5][o], u2 = B[6][o];
+          P1[1] = fma(w, ru, P1[1]);
+          P1[2] = fma(w, rv, P1[2]);
+          P1[3] = fma(w, rw, P1[3]);
+          P1[4] = fma(w, ru * u0, P1[4]);
+          P1[5] = fma(w, ru * u1, P1[5]);
+          P1[6] = fma(w, ru * u2, P1[6]);
+          P1[7] = fma(w, rv * u1, P1[7]);
+          P1[8] = fma(w, rv * u2, P1[8]);
+          P1[9] = fma(w, rw * u2, P1[9]);
+        } else {
+          const T rS = rho * B[7][o];
 #pragma unroll
-        for (int c = 0; c < 6; c++) {
-          const T Sv = B[7 + c][o];
-          P1[10 + c] = fma(w, Sv, P1[10 + c]);
-          P1[16 + c] = fma(w, Sv * rS, P1[16 + c]);
+          for (int c = 0; c < 6; c++) {
+            const T Sv = B[1 + c][o];
+            P1[1 + c] = fma(w, Sv, P1[1 + c]);
+            P1[7 + c] = fma(w, Sv * rS, P1[7 + c]);
+          }
         }
       }
-    T Sx1[6];
+    if (PART == 1) {
 #pragma unroll
-    for (int c = 0; c < 6; c++) Sx1[c] = B[7 + c][c0];
-    const bool act1 = col && kk >= 1 && kk <= g.mz - 2 && !(nvat(a, nid(g, kk, j, i)) > 1.1);
+      for (int c = 0; c < 6; c++) Sx1[c] = B[1 + c][c0];
+    }
     __syncthreads();  // the next plane overwrites the buffer
-    // ---- node (kk - 1, j, i): its 27-point sums, L_ij and M_ij
+    // ---- node (kk - 1, j, i): its 27-point sums, L_ij or M_ij
     const int k = kk - 1;
     if (col && k >= k0) {
       const long long n0 = nid(g, k, j, i);
+      T *out = PART == 0 ? a.lm : a.mm;
       if (!act0) {
-        a.lm[n0] = T(0);
-        a.mm[n0] = T(0);
+        out[n0] = T(0);
       } else {
-        T sm[FNS];
+        T sm[NS];
 #pragma unroll
-        for (int q = 0; q < FNS; q++) sm[q] = fma(T(0.5), Pm[q], P0[q]) + T(0.5) * P1[q];
+        for (int q = 0; q < NS; q++) sm[q] = fma(T(0.5), Pm[q], P0[q]) + T(0.5) * P1[q];
         const T iw = T(FILTER_INV_WSUM);
-        const T _rho = sm[0] * iw, _rhou = sm[1] * iw, _rhov = sm[2] * iw, _rhow = sm[3] * iw;
-        const T irho = T(1) / _rho;
+        const T _rho = sm[0] * iw;
         // full 3x3 tensors in row-major order from the six symmetric components
         const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-        T Lij[9];
-        Lij[0] = sm[4] * iw - _rhou * _rhou * irho;
-        Lij[1] = sm[5] * iw - _rhou * _rhov * irho;
-        Lij[2] = sm[6] * iw - _rhou * _rhow * irho;
-        Lij[3] = sm[5] * iw - _rhov * _rhou * irho;
-        Lij[4] = sm[7] * iw - _rhov * _rhov * irho;
-        Lij[5] = sm[8] * iw - _rhov * _rhow * irho;
-        Lij[6] = sm[6] * iw - _rhow * _rhou * irho;
-        Lij[7] = sm[8] * iw - _rhow * _rhov * irho;
-        Lij[8] = sm[9] * iw - _rhow * _rhow * irho;
-        T Sh6[6], SSh6[6];
+        if (PART == 0) {
+          const T _rhou = sm[1] * iw, _rhov = sm[2] * iw, _rhow = sm[3] * iw;
+          const T irho = T(1) / _rho;
+          T Lij[9];
+          Lij[0] = sm[4] * iw - _rhou * _rhou * irho;
+          Lij[1] = sm[5] * iw - _rhou * _rhov * irho;
+          Lij[2] = sm[6] * iw - _rhou * _rhow * irho;
+          Lij[3] = sm[5] * iw - _rhov * _rhou * irho;
+          Lij[4] = sm[7] * iw - _rhov * _rhov * irho;
+          Lij[5] = sm[8] * iw - _rhov * _rhow * irho;
+          Lij[6] = sm[6] * iw - _rhow * _rhou * irho;
+          Lij[7] = sm[8] * iw - _rhow * _rhov * irho;
+          Lij[8] = sm[9] * iw - _rhow * _rhow * irho;
+          T L_cat = 0;
 #pragma unroll
-        for (int c = 0; c < 6; c++) {
-          Sh6[c] = sm[10 + c] * iw;
-          SSh6[c] = sm[16 + c] * iw;
-        }
-        T S_hat = 0;
+          for (int p = 0; p < 3; p++)
 #pragma unroll
-        for (int c = 0; c < 9; c++) S_hat += Sh6[sym[c]] * Sh6[sym[c]];
-        S_hat = sqrt(T(2) * S_hat);
-        T L_cat = 0, M_cat = 0;
-        const T tf2 = T(a.tfil2[n0]), f2 = T(a.fil2[n0]);
-        const T m1 = T(-2.0) * _rho * tf2 * S_hat, m2 = T(2.0) * f2;
+            for (int q = 0; q < 3; q++) {
+              const int ab = sym[p * 3 + q], aa = sym[p * 4];
+              L_cat += Lij[p * 3 + q] * Sx0[ab] - T(0.33) * Sx0[aa] * Lij[q * 4];
+            }
+          out[n0] = L_cat;
+        } else {
+          T Sh6[6], SSh6[6];
 #pragma unroll
-        for (int p = 0; p < 3; p++)
-#pragma unroll
-          for (int q = 0; q < 3; q++) {
-            const int ab = sym[p * 3 + q], aa = sym[p * 4], bb = sym[q * 4];
-            M_cat += m1 * (Sh6[ab] * Sx0[ab] - T(0.33) * Sh6[aa] * Sx0[bb]) +
-                     m2 * (SSh6[ab] * Sx0[ab] - T(0.33) * SSh6[aa] * Sx0[bb]);
-            L_cat += Lij[p * 3 + q] * Sx0[ab] - T(0.33) * Sx0[aa] * Lij[q * 4];
+          for (int c = 0; c < 6; c++) {
+            Sh6[c] = sm[1 + c] * iw;
+            SSh6[c] = sm[7 + c] * iw;
           }
-        a.lm[n0] = L_cat;
-        a.mm[n0] = M_cat;
+          T S_hat = 0;
+#pragma unroll
+          for (int c = 0; c < 9; c++) S_hat += Sh6[sym[c]] * Sh6[sym[c]];
+          S_hat = sqrt(T(2) * S_hat);
+          T M_cat = 0;
+          const T tf2 = T(a.tfil2[n0]), f2 = T(a.fil2[n0]);
+          const T m1 = T(-2.0) * _rho * tf2 * S_hat, m2 = T(2.0) * f2;
+#pragma unroll
+          for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+              const int ab = sym[p * 3 + q], aa = sym[p * 4], bb = sym[q * 4];
+              M_cat += m1 * (Sh6[ab] * Sx0[ab] - T(0.33) * Sh6[aa] * Sx0[bb]) +
+                       m2 * (SSh6[ab] * Sx0[ab] - T(0.33) * SSh6[aa] * Sx0[bb]);
+            }
+          out[n0] = M_cat;
+        }
       }
     }
 #pragma unroll
-    for (int q = 0; q < FNS; q++) {
+    for (int q = 0; q < NS; q++) {
       Pm[q] = P0[q];
       P0[q] = P1[q];
     }
@@ -669,7 +705,9 @@ int launch_les_t(cles_ctx *c, hipStream_t stream, const double *const *qs_cell) 
     const int ntx = (c->g.mx - 2 + FTX - 1) / FTX, nty = (c->g.my - 2 + FTY - 1) / FTY,
               ntz = (c->g.mz - 2 + FKC - 1) / FKC;
     const int ntiles = ntx * nty * ntz, pxf = (ntiles + 7) / 8;
-    hipLaunchKernelGGL(les_filter_kernel<T>, dim3(8 * pxf), dim3(256), 0, stream, a, ntx, nty, ntiles, pxf);
+    hipLaunchKernelGGL((les_filter_kernel<T, 0>), dim3(8 * pxf), dim3(256), 0, stream, a, ntx, nty, ntiles, pxf);
+    CLES_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL((les_filter_kernel<T, 1>), dim3(8 * pxf), dim3(256), 0, stream, a, ntx, nty, ntiles, pxf);
   }
   CLES_HIP(c, hipGetLastError());
   if (a.ih) {  // the averaged branches (-j or -k alone average nothing)

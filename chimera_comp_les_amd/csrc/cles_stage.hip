@@ -903,12 +903,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   const bool les_e = LES && VISC && valid && gj > 2 && gj < g.my - 2 && !(a.has_ib && !(a.nv[n] < 0.5));
   T E0 = T(0), E1 = T(0), E2 = T(0);
   Win win{i0 - 1, j0 - 1, k0 - 1};
+  constexpr int NIT = (NW + TL::NT - 1) / TL::NT;
+  double qv[NIT][5];
+  bool ok[NIT];
   if (VISC) {
     // stage u, v, w, p/rho of the tile and a one-node halo; the loads of all
     // slots are issued before any LDS store, so they are in flight together
-    constexpr int NIT = (NW + TL::NT - 1) / TL::NT;
-    double qv[NIT][5];
-    bool ok[NIT];
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
       const int w = t + it * TL::NT;
@@ -921,6 +921,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         qv[it][4] = ldo(a.p, mo);
       }
     }
+  }
+  // the window's LDS stores: in the staged form they follow the first
+  // direction's loads, so both load phases are in flight together
+  auto store_window = [&]() {
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
       const int w = t + it * TL::NT;
@@ -931,6 +935,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
       Ws[2 * NW + w] = T(qv[it][3] * ir);
       Ws[3 * NW + w] = T(qv[it][4] * ir - a.t_ref);  // t_ref = 0 in FP64: only gradients of p/rho are used
     }
+  };
+  if (VISC && !STG) {
+    store_window();
     __syncthreads();  // the window is read by other threads' viscous faces
   }
 #pragma unroll
@@ -1003,6 +1010,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
           }
         }
       }
+      if (VISC && d == 0) store_window();
 #pragma unroll
       for (int it = 0; it < NBI; it++) {
         if (!bok[it]) continue;
