@@ -219,7 +219,7 @@ size_t ctx_footprint(const cles_params *p) {
   size_t planes = 3 + 9 + 1 + 1 + 1 + 15 + 3 + 5;  // coor, metrics, aj, nvert, fil2, Q x3, P x3, rhs
   size_t extra = 0;  // doubles outside whole planes
   if (p->les) {
-    planes += 1 + 3;  // tfil2, cs, cs_alt, nut
+    planes += 1 + 3 + 10;  // tfil2, cs, cs_alt, nut, packed metrics
     if (p->calc_sigma) planes += 6;  // shock sensor and its smoothing buffer
     if (p->les != 1 && p->les != 10) {
       planes += 1;  // 1/aj
@@ -296,6 +296,7 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
   TRY(dalloc(c, &c->aj, np));
   TRY(dalloc(c, &c->nvert, np));
   TRY(dalloc(c, &c->fil2, np));
+  if (p->les) TRY(dalloc(c, &c->mpack, 10 * np));
   for (int b = 0; b < 3; b++) {
     TRY(dalloc(c, &c->Q[b], 5 * np));
     TRY(dalloc(c, &c->P[b], np));
@@ -343,7 +344,7 @@ void cles_destroy(cles_ctx *c) {
   member_leave(c);  // waits for its batch, which then rebuilds without it
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->qsnap, c->tfil2, c->ivol, c->sabs, c->avg, c->sgm, c->sgm_tmp, c->homo};
+                    c->fil2, c->qsnap, c->tfil2, c->ivol, c->sabs, c->avg, c->sgm, c->sgm_tmp, c->homo, c->mpack};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   ibm_free(c);

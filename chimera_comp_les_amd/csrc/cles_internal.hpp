@@ -253,6 +253,7 @@ struct cles_ctx {
   double *csi, *eta, *zet;  // 3 planes each
   double *aj, *nvert;       // 1 plane each
   double *fil2;             // (pow(1/aj, 1/3))^2: squared LES filter width per node
+  double *mpack;            // LES: csi, eta, zet and aj in blocks of 8 nodes, [np/8][5][8][2]
   double *tfil2;            // squared test-filter width (2 or 5^(1/3) filter widths)
   double ref_rho, ref_p;    // FP32-bulk reference state (node (1,1,1) of the uploaded state)
   int tile;                 // stage-kernel tile shape 0..7 (cles_set_tile), default 7: 8x4x3 in 128 threads
@@ -291,7 +292,7 @@ struct cles_ctx {
 
 namespace cles {
 // launchers implemented in the .hip translation units
-int launch_metrics(cles_ctx *c);
+int launch_metrics(cles_ctx *c);  // also fills mpack when the context has one
 int launch_eos_all(cles_ctx *c, int buf);
 int launch_stage(cles_ctx *c, int in, int out, int istage, int mode, const double coef[3]);
 int launch_boundary(cles_ctx *c, int in, int out, int istage, double alfa, const double coef[3]);

@@ -110,6 +110,30 @@ __global__ void metric_copy_kernel(CopyArgs a) {
 // squared LES filter width (pow(1/aj, 1/3))^2 of rhs.c:1693 and les.c:1574,
 // formed once per node with the same two pow calls the reference makes
 
+// The nine metrics and aj (values 0..8 the csi, eta, zet rows, 9 aj) packed
+// in blocks of 8 consecutive nodes: block b holds five 128-byte lines, line p
+// the pairs (2p, 2p + 1) of its 8 nodes, so value c of node n sits at
+// ((n / 8) * 5 + c / 2) * 16 + (n % 8) * 2 + c % 2.  A face reads the ten
+// values of each of its nodes with five 16-byte loads, a box node its metric
+// row with two, and neighbouring nodes share the lines as in the planes.  The
+// planes stay the metrics every other kernel reads.
+struct PackArgs {
+  Grid g;
+  const double *m[4];  // csi, eta, zet (3 planes each), aj
+  double *pack;
+};
+__global__ void metric_pack_kernel(PackArgs a) {
+  const Grid &g = a.g;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long N = (long long)g.mx * g.my * g.mz, np = g.np;
+  if (t >= N) return;
+  const int i = (int)(t % g.mx), j = (int)((t / g.mx) % g.my), k = (int)(t / ((long long)g.mx * g.my));
+  const long long n = nid(g, k, j, i);
+#pragma unroll
+  for (int c = 0; c < 10; c++)
+    a.pack[((n >> 3) * 5 + (c >> 1)) * 16 + (n & 7) * 2 + (c & 1)] = c < 9 ? a.m[c / 3][(c % 3) * np + n] : a.m[3][n];
+}
+
 }  // namespace
 
 int launch_metrics(cles_ctx *c) {
@@ -125,6 +149,11 @@ int launch_metrics(cles_ctx *c) {
       hipLaunchKernelGGL(metric_copy_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, ca);
       CLES_HIP(c, hipGetLastError());
     }
+  if (c->mpack) {  // after the boundary copies, which complete the metric planes
+    PackArgs pa{c->g, {c->csi, c->eta, c->zet, c->aj}, c->mpack};
+    hipLaunchKernelGGL(metric_pack_kernel, dim3((unsigned)((N + bs - 1) / bs)), dim3(bs), 0, c->stream, pa);
+    CLES_HIP(c, hipGetLastError());
+  }
   return CLES_OK;
 }
 

@@ -76,6 +76,7 @@ struct StageArgs {
   double *qo, *po;      // stage output
   double *rhs;          // parity mode output (5 planes)
   const double *csi, *eta, *zet, *aj, *nv, *nut, *fil2;
+  const double *mpack;  // LES: the nine metrics and aj in blocks of 8 nodes ([np/8][5][8][2])
   int bc[6];
   int fluxorder, eno, central, iso, has_ib;
   double ren, Ma, Pr, gamma, Tref, ratio, dt;
@@ -115,6 +116,12 @@ __device__ __forceinline__ void load6(const StageArgs &a, long long n, T s[6]) {
 // instructions, instead of forming a 64-bit address per access.
 __device__ __forceinline__ unsigned node_off(const Grid &g, int k, int j, int i) {
   return ((unsigned)k * (unsigned)g.sz + (unsigned)j * (unsigned)g.sy + (unsigned)i) * 8u;
+}
+// first value of a node in the blocked metric pack (value pair p of the node
+// at + 16 p), from the node's byte offset in a plane
+__device__ __forceinline__ const double *mpack_at(const double *pack, unsigned off) {
+  const long long n = (long long)(off >> 3);
+  return pack + (n >> 3) * 80 + (n & 7) * 2;
 }
 __device__ __forceinline__ double ldo(const double *plane, unsigned off) {
   return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(plane) + off);
@@ -865,11 +872,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   constexpr int NS0 = (TX + 4) * TY * TZ, NS1 = TX * (TY + 4) * TZ, NS2 = TX * TY * (TZ + 4);
   constexpr int NSM = NS0 > NS1 ? (NS0 > NS2 ? NS0 : NS2) : (NS1 > NS2 ? NS1 : NS2);
   // Staged form (LES, every face of a direction in one pass of the
-  // workgroup): per direction, one load phase fills the node box (state, p,
-  // contravariant velocity) and the face buffer FM (the face's nine metric
-  // averages and aj, later its viscous coefficients), so the face passes read
-  // LDS only; the flux buffer Fs then reuses the box's memory.  Elsewhere
-  // (larger tiles) the face routines read global memory.
+  // workgroup): per direction, one load phase fills the node box in LDS
+  // (state, p, contravariant velocity from the metric row of the blocked
+  // metric pack) and gives each face's thread its nine metric averages and aj
+  // (from the pack) and viscous inputs in registers, later its viscous
+  // coefficients, so the face passes read LDS and registers only; the flux
+  // buffer Fs then reuses the box's memory.  Elsewhere (larger tiles) the
+  // face routines read global memory.
   constexpr bool ONEPASS = TL::NFI <= TL::NT && TL::NFJ <= TL::NT && TL::NFK <= TL::NT;
   constexpr int NFM = VISC ? 12 : 3;
   constexpr int NSF = NCH * NSM > FC * NFD ? NCH * NSM : FC * NFD;
@@ -977,8 +986,24 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
 #pragma unroll
           for (int c = 0; c < 5; c++) bv[it][c] = ldo(a.q + c * np, mo);
           bv[it][5] = ldo(a.p, mo);
-#pragma unroll
-          for (int c = 0; c < 3; c++) bv[it][6 + c] = ldo(mtd + c * np, mo);
+          // metric row d (values 3d .. 3d + 2) from the pack: two loads
+          const double *pk = mpack_at(a.mpack, mo);
+          if (d == 0) {
+            const double2 x = *reinterpret_cast<const double2 *>(pk);
+            bv[it][6] = x.x;
+            bv[it][7] = x.y;
+            bv[it][8] = pk[16];
+          } else if (d == 1) {
+            bv[it][6] = pk[17];
+            const double2 x = *reinterpret_cast<const double2 *>(pk + 32);
+            bv[it][7] = x.x;
+            bv[it][8] = x.y;
+          } else {
+            const double2 x = *reinterpret_cast<const double2 *>(pk + 48);
+            bv[it][6] = x.x;
+            bv[it][7] = x.y;
+            bv[it][8] = pk[64];
+          }
         }
       }
       const bool fload = fval && !skip_d;
@@ -988,14 +1013,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         const unsigned oA = node_off(g, fk, fj, fi);
         const unsigned oB = oA + 8u * (unsigned)(d == 0 ? 1 : d == 1 ? g.sy : g.sz);
         if (VISC) {
-          const double *mt[3] = {a.csi, a.eta, a.zet};
+          // the nine metrics and aj of nodes A and B from the metric pack:
+          // five 16-byte loads per node
+          const double2 *pA = reinterpret_cast<const double2 *>(mpack_at(a.mpack, oA));
+          const double2 *pB = reinterpret_cast<const double2 *>(mpack_at(a.mpack, oB));
 #pragma unroll
-          for (int c = 0; c < 9; c++) {
-            fm[c][0] = ldo(mt[c / 3] + (c % 3) * np, oA);
-            fm[c][1] = ldo(mt[c / 3] + (c % 3) * np, oB);
+          for (int c = 0; c < 5; c++) {
+            const double2 xa = pA[8 * c], xb = pB[8 * c];
+            fm[2 * c][0] = xa.x;
+            fm[2 * c + 1][0] = xa.y;
+            fm[2 * c][1] = xb.x;
+            fm[2 * c + 1][1] = xb.y;
           }
-          fm[9][0] = ldo(a.aj, oA);
-          fm[9][1] = ldo(a.aj, oB);
           if (LES) {
             fin[0] = ldo(a.nut, oA);
             fin[1] = ldo(a.nut, oB);
@@ -1511,6 +1540,7 @@ void fill_stage_args(cles_ctx *c, int in, int out, int istage, const double coef
   a.rhs = c->rhs;
   a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert; a.nut = c->nut;
   a.fil2 = c->fil2;
+  a.mpack = c->mpack;
   for (int q = 0; q < 6; q++) a.bc[q] = p.bctype[q];
   a.fluxorder = p.fluxorder; a.eno = p.eno_order; a.central = p.central; a.iso = p.isothermal;
   a.has_ib = c->has_ib;
