@@ -186,31 +186,34 @@ __device__ __forceinline__ long long seam_image(const Grid &g, int k, int j, int
 // the 27-point weighted sum of a quantity is 0.5 P(k-1) + P(k) + 0.5 P(k+1)
 // with P(k) the 9-point weighted sum over plane k; the weights sum to 8, so
 // vsum / wsum is vsum * 0.125.  A workgroup owns a 16 x 16 column of
-// interior nodes and walks a chunk of planes: each plane's node data with a
-// one-node halo is staged in LDS once, every thread forms the plane sums of
-// its node from it, and keeps the sums of the two planes before in
-// registers.  The contraction runs as two launches that share no data, each
-// holding about half the sums, so that each fits 128 VGPRs (4 waves per SIMD;
-// one launch with all 22 sums needs 221 and runs at 2):
-//   PART 0, L_ij S_ij: sums of rho, rhoU, rhoV, rhoW and the six rho u_a u_b
+// interior nodes and walks a chunk of kc planes: each plane's node data with
+// a one-node halo is staged in LDS once, every thread forms the plane sums
+// of its node from it, and keeps the sums of the two planes before in
+// registers.  The node data and sums of the contraction:
+//   L_ij S_ij: sums of rho, rhoU, rhoV, rhoW and the six rho u_a u_b
 //     products (staged: rho, rhoU, rhoV, rhoW and the velocity lUcat holds
 //     after the gradient loop); writes LM;
-//   PART 1, M_ij S_ij: sums of rho, the six strain-rate components as the
-//     filter sees them (periodic face copies of lSx..lSz, les.c:316-377, zero
-//     on other boundary nodes) and their products with rho |S| (|S| without a
+//   M_ij S_ij: sums of rho, the six strain-rate components as the filter
+//     sees them (periodic face copies of lSx..lSz, les.c:316-377, zero on
+//     other boundary nodes) and their products with rho |S| (|S| without a
 //     periodic copy); writes MM.
-// Each sum is formed with the same operations, in the same order, as one
-// launch would form it.
+// PART 2 forms both (22 sums); PART 0 the L part and PART 1 the M part, two
+// launches that share no data.  In double the 22 sums need 221 VGPRs (2
+// waves per SIMD) and each part fits 128 (4 waves); in float all 22 fit 128.
+// Each sum is formed with the same operations, in the same order, in every
+// part.
 constexpr int FTX = 16, FTY = 16;
 constexpr int FBX = FTX + 2, FBY = FTY + 2, FNB = FBX * FBY;
-constexpr int FKC = 32;  // planes per workgroup
 constexpr double FILTER_INV_WSUM = 0.125;
 static_assert(FTX * FTY == 256, "one column per thread");
 
 template <typename T, int PART>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void les_filter_kernel(LesArgs<T> a, int ntx, int nty, int ntiles, int per_xcd) {
-  constexpr int NQ = PART == 0 ? 7 : 8;    // staged values per node
-  constexpr int NS = PART == 0 ? 10 : 13;  // plane sums per node
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void les_filter_kernel(LesArgs<T> a, int ntx, int nty, int kc, int ntiles, int per_xcd) {
+  constexpr bool DL = PART != 1, DM = PART != 0;  // forms L_ij S_ij, M_ij S_ij
+  constexpr int QS = DL ? 7 : 1;                   // first staged strain-rate component
+  constexpr int NQ = DM ? QS + 7 : 7;              // staged values per node
+  constexpr int PS = DL ? 10 : 1;                  // first strain-rate sum
+  constexpr int NS = DM ? PS + 12 : 10;            // plane sums per node
   __shared__ T B[NQ][FNB];
   const Grid &g = a.g;
   const long long np = g.np;
@@ -218,8 +221,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
   if (tile >= ntiles) return;
   // x fastest, then y, then the plane chunk
   const int i0 = 1 + (tile % ntx) * FTX, j0 = 1 + ((tile / ntx) % nty) * FTY;
-  const int k0 = 1 + (tile / (ntx * nty)) * FKC;
-  const int k1 = k0 + FKC - 1 < g.mz - 2 ? k0 + FKC - 1 : g.mz - 2;  // last plane of the chunk
+  const int k0 = 1 + (tile / (ntx * nty)) * kc;
+  const int k1 = k0 + kc - 1 < g.mz - 2 ? k0 + kc - 1 : g.mz - 2;  // last plane of the chunk
   const int t = threadIdx.x;
   const int lx = t % FTX, ly = t / FTX;
   const int i = i0 + lx, j = j0 + ly;
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
       const long long m = nid(g, kk, jj, ii);
       const double r = a.q[m];
       B[0][w] = T(r);
-      if (PART == 0) {
+      if (DL) {
         const double ru = a.q[m + np], rv = a.q[m + 2 * np], rw = a.q[m + 3 * np];
         B[1][w] = T(ru);
         B[2][w] = T(rv);
@@ -252,21 +255,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
         B[4][w] = u[0];
         B[5][w] = u[1];
         B[6][w] = u[2];
-      } else {
+      }
+      if (DM) {
         long long ms = m;
         T S = T(0);
         if (interior(g, kk, jj, ii)) S = a.grad[m + 6 * np];
         else ms = seam_image(g, kk, jj, ii);
 #pragma unroll
-        for (int c = 0; c < 6; c++) B[1 + c][w] = ms >= 0 ? a.grad[ms + c * np] : T(0);
-        B[7][w] = S;
+        for (int c = 0; c < 6; c++) B[QS + c][w] = ms >= 0 ? a.grad[ms + c * np] : T(0);
+        B[QS + 6][w] = S;
       }
     }
     const bool act1 = col && kk >= 1 && kk <= g.mz - 2 && !(nvat(a, nid(g, kk, j, i)) > 1.1);
-    // the node's own strain rates (it is interior when active): read here,
-    // used when the node's sums are complete, one plane later
+    // the node's own strain rates (it is interior when active): without the
+    // M part they are read here, and used when the node's sums are complete,
+    // one plane later
     T Sx1[6];
-    if (PART == 0) {
+    if (!DM) {
       const long long n1 = act1 ? nid(g, kk, j, i) : 0;
 #pragma unroll
       for (int c = 0; c < 6; c++) Sx1[c] = act1 ? a.grad[n1 + c * np] : T(0);
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
         const T w = T((Qd == 1 ? 1.0 : 0.5) * (Pd == 1 ? 1.0 : 0.5));
         const T rho = B[0][o];
         P1[0] = fma(w, rho, P1[0]);
-        if (PART == 0) {
+        if (DL) {
           const T ru = B[1][o], rv = B[2][o], rw = B[3][o];
           const T u0 = B[4][o], u1 = B[5][o], u2 = B[6][o];
           P1[1] = fma(w, ru, P1[1]);
@@ -298,28 +303,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
           P1[7] = fma(w, rv * u1, P1[7]);
           P1[8] = fma(w, rv * u2, P1[8]);
           P1[9] = fma(w, rw * u2, P1[9]);
-        } else {
-          const T rS = rho * B[7][o];
+        }
+        if (DM) {
+          const T rS = rho * B[QS + 6][o];
 #pragma unroll
           for (int c = 0; c < 6; c++) {
-            const T Sv = B[1 + c][o];
-            P1[1 + c] = fma(w, Sv, P1[1 + c]);
-            P1[7 + c] = fma(w, Sv * rS, P1[7 + c]);
+            const T Sv = B[QS + c][o];
+            P1[PS + c] = fma(w, Sv, P1[PS + c]);
+            P1[PS + 6 + c] = fma(w, Sv * rS, P1[PS + 6 + c]);
           }
         }
       }
-    if (PART == 1) {
+    if (DM) {
 #pragma unroll
-      for (int c = 0; c < 6; c++) Sx1[c] = B[1 + c][c0];
+      for (int c = 0; c < 6; c++) Sx1[c] = B[QS + c][c0];
     }
     __syncthreads();  // the next plane overwrites the buffer
-    // ---- node (kk - 1, j, i): its 27-point sums, L_ij or M_ij
+    // ---- node (kk - 1, j, i): its 27-point sums, L_ij and M_ij
     const int k = kk - 1;
     if (col && k >= k0) {
       const long long n0 = nid(g, k, j, i);
-      T *out = PART == 0 ? a.lm : a.mm;
       if (!act0) {
-        out[n0] = T(0);
+        if (DL) a.lm[n0] = T(0);
+        if (DM) a.mm[n0] = T(0);
       } else {
         T sm[NS];
 #pragma unroll
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
         const T _rho = sm[0] * iw;
         // full 3x3 tensors in row-major order from the six symmetric components
         const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-        if (PART == 0) {
+        if (DL) {
           const T _rhou = sm[1] * iw, _rhov = sm[2] * iw, _rhow = sm[3] * iw;
           const T irho = T(1) / _rho;
           T Lij[9];
@@ -349,13 +355,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
               const int ab = sym[p * 3 + q], aa = sym[p * 4];
               L_cat += Lij[p * 3 + q] * Sx0[ab] - T(0.33) * Sx0[aa] * Lij[q * 4];
             }
-          out[n0] = L_cat;
-        } else {
+          a.lm[n0] = L_cat;
+        }
+        if (DM) {
           T Sh6[6], SSh6[6];
 #pragma unroll
           for (int c = 0; c < 6; c++) {
-            Sh6[c] = sm[1 + c] * iw;
-            SSh6[c] = sm[7 + c] * iw;
+            Sh6[c] = sm[PS + c] * iw;
+            SSh6[c] = sm[PS + 6 + c] * iw;
           }
           T S_hat = 0;
 #pragma unroll
@@ -372,7 +379,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
               M_cat += m1 * (Sh6[ab] * Sx0[ab] - T(0.33) * Sh6[aa] * Sx0[bb]) +
                        m2 * (SSh6[ab] * Sx0[ab] - T(0.33) * SSh6[aa] * Sx0[bb]);
             }
-          out[n0] = M_cat;
+          a.mm[n0] = M_cat;
         }
       }
     }
@@ -702,12 +709,19 @@ int launch_les_t(cles_ctx *c, hipStream_t stream, const double *const *qs_cell) 
   hipLaunchKernelGGL(les_grad_kernel<T>, dim3(gX), dim3(bs), 0, stream, a, per_xcd);
   CLES_HIP(c, hipGetLastError());
   {
-    const int ntx = (c->g.mx - 2 + FTX - 1) / FTX, nty = (c->g.my - 2 + FTY - 1) / FTY,
-              ntz = (c->g.mz - 2 + FKC - 1) / FKC;
-    const int ntiles = ntx * nty * ntz, pxf = (ntiles + 7) / 8;
-    hipLaunchKernelGGL((les_filter_kernel<T, 0>), dim3(8 * pxf), dim3(256), 0, stream, a, ntx, nty, ntiles, pxf);
-    CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL((les_filter_kernel<T, 1>), dim3(8 * pxf), dim3(256), 0, stream, a, ntx, nty, ntiles, pxf);
+    // chunks of 32 planes, or fewer where that leaves fewer workgroups than
+    // the chip holds at once (256 CUs x 4): a chunk is a serial march
+    const int ntx = (c->g.mx - 2 + FTX - 1) / FTX, nty = (c->g.my - 2 + FTY - 1) / FTY;
+    int kc = 32;
+    while (kc > 4 && ntx * nty * ((c->g.mz - 2 + kc - 1) / kc) < 1024) kc /= 2;
+    const int ntiles = ntx * nty * ((c->g.mz - 2 + kc - 1) / kc), pxf = (ntiles + 7) / 8;
+    if constexpr (sizeof(T) == 8) {
+      hipLaunchKernelGGL((les_filter_kernel<T, 0>), dim3(8 * pxf), dim3(256), 0, stream, a, ntx, nty, kc, ntiles, pxf);
+      CLES_HIP(c, hipGetLastError());
+      hipLaunchKernelGGL((les_filter_kernel<T, 1>), dim3(8 * pxf), dim3(256), 0, stream, a, ntx, nty, kc, ntiles, pxf);
+    } else {
+      hipLaunchKernelGGL((les_filter_kernel<T, 2>), dim3(8 * pxf), dim3(256), 0, stream, a, ntx, nty, kc, ntiles, pxf);
+    }
   }
   CLES_HIP(c, hipGetLastError());
   if (a.ih) {  // the averaged branches (-j or -k alone average nothing)
