@@ -911,6 +911,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   // LES energy term (rhs.c:2153-2197): face-density weighted alpha sums
   const bool les_e = LES && VISC && valid && gj > 2 && gj < g.my - 2 && !(a.has_ib && !(a.nv[n] < 0.5));
   T E0 = T(0), E1 = T(0), E2 = T(0);
+  // the node's update inputs; the staged form takes them during the last
+  // direction (the state from its box, aj and Qold loaded with its box)
+  double ajn = 0., qin[5] = {0., 0., 0., 0., 0.}, qol[5] = {0., 0., 0., 0., 0.};
+  const unsigned no = (unsigned)n * 8u;
+  auto load_update = [&]() {
+    if (!valid) return;
+    ajn = ldo(a.aj, no);
+    if (MODE != 0) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) qol[q] = ldo(a.qold + q * np, no);
+    }
+  };
   Win win{i0 - 1, j0 - 1, k0 - 1};
   constexpr int NIT = (NW + TL::NT - 1) / TL::NT;
   double qv[NIT][5];
@@ -1039,6 +1051,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
           }
         }
       }
+      if (d == 2) load_update();
       if (VISC && d == 0) store_window();
 #pragma unroll
       for (int it = 0; it < NBI; it++) {
@@ -1082,6 +1095,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
         const double r0 = Ss[on];
         rl = T((r0 + Ss[on - bst]) * 0.5);
         rh = T((Ss[on + bst] + r0) * 0.5);
+      }
+      if (d == 2 && valid) {
+        const int on = ((gk - oz) * by + (gj - oy)) * bx + (gi - ox);
+#pragma unroll
+        for (int q = 0; q < 5; q++) qin[q] = Ss[q * ns + on];
       }
       __syncthreads();  // the box is read: the flux buffer takes its memory
       if (fval) {
@@ -1222,17 +1240,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
     conv[1] = 0.;
     visc[1] = 0.;
   }
-  // the node's update inputs
-  double ajn = 0., qin[5] = {0., 0., 0., 0., 0.}, qol[5] = {0., 0., 0., 0., 0.};
-  const unsigned no = (unsigned)n * 8u;
-  if (valid) {
-    ajn = ldo(a.aj, no);
+  if (!STG && valid) {
+    load_update();
 #pragma unroll
     for (int q = 0; q < 5; q++) qin[q] = ldo(a.q + q * np, no);
-    if (MODE != 0) {
-#pragma unroll
-      for (int q = 0; q < 5; q++) qol[q] = ldo(a.qold + q * np, no);
-    }
   }
   if (LES && a.ratio != 0.0 && valid) {
     conv[3] = conv[3] - a.ratio / ajn * qin[0] / a.dt;
