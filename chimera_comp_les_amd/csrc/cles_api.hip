@@ -630,16 +630,20 @@ int cles_time_stage_kernel(cles_ctx *c, int nrep, double *ms) {
   if (int g_ = member_guard(c, true)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
   const int out = free_buffer(c, c->cur, c->ucat_src);
+  // input: the last stage's input where a step has run, so that the timed
+  // launch reads Qold as stages after the first do (the first reads its own
+  // input as Qold); else the current state
+  const int in = c->ucat_src >= 0 && c->ucat_src != c->cur ? c->ucat_src : c->cur;
   double coef[3], alfa;
   rk_coefficients(c->p, c->p.fluxorder > 1 ? 1 : 0, coef, &alfa);
   hipEvent_t e0, e1;
   CLES_HIP(c, hipEventCreate(&e0));
   CLES_HIP(c, hipEventCreate(&e1));
-  int r = launch_stage(c, c->cur, out, 1, 1, coef);  // warm-up
+  int r = launch_stage(c, in, out, 1, 1, coef);  // warm-up
   if (r) return r;
   CLES_HIP(c, hipEventRecord(e0, c->stream));
   for (int q = 0; q < nrep; q++) {
-    r = launch_stage(c, c->cur, out, 1, 1, coef);
+    r = launch_stage(c, in, out, 1, 1, coef);
     if (r) return r;
   }
   CLES_HIP(c, hipEventRecord(e1, c->stream));

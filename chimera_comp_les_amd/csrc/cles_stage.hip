@@ -915,10 +915,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   // direction (the state from its box, aj and Qold loaded with its box)
   double ajn = 0., qin[5] = {0., 0., 0., 0., 0.}, qol[5] = {0., 0., 0., 0., 0.};
   const unsigned no = (unsigned)n * 8u;
+  // Qold is the stage input itself in the first stage: its values are qin
+  const bool qold_in = a.qold == a.q;
   auto load_update = [&]() {
     if (!valid) return;
     ajn = ldo(a.aj, no);
-    if (MODE != 0) {
+    if (MODE != 0 && !qold_in) {
 #pragma unroll
       for (int q = 0; q < 5; q++) qol[q] = ldo(a.qold + q * np, no);
     }
@@ -1289,6 +1291,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &a, int tile, int b) 
   }
   // ---- stage update, VecWAXPY / VecAXPBYPCZ (solvers.c:256-271)
   double qn[5];
+  if (qold_in) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) qol[q] = qin[q];
+  }
   if (a.istage == 0) {
 #pragma unroll
     for (int q = 0; q < 5; q++) qn[q] = a.a_rhs * rhs[q] + qol[q];

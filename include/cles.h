@@ -230,9 +230,11 @@ int cles_get_time(cles_ctx *c);
 int cles_set_time(cles_ctx *c, int ti);
 
 /* Kernel timing for measurement: runs the fused stage kernel nrep times
- * back to back on the context's stream (stage-1 update of the current state
- * into a scratch buffer, so the state is unchanged) between two HIP events
- * and returns the mean duration per launch in milliseconds. */
+ * back to back on the context's stream (a stage-1 update into a scratch
+ * buffer, so the state is unchanged; its input is the last stage's input
+ * once a step has run, else the current state, with the current state as
+ * Qold) between two HIP events and returns the mean duration per launch in
+ * milliseconds. */
 int cles_time_stage_kernel(cles_ctx *c, int nrep, double *ms);
 /* Same for one LES model update (all of its kernels); the LES fields are
  * recomputed from the current state, nothing else changes. */
