@@ -25,6 +25,8 @@
 // float for the FP32-bulk form (the model is not on the list of quantities
 // that form keeps in FP64); Cs and nu_t are stored in double for the stage
 // kernel.
+#include <type_traits>
+
 #include "cles_internal.hpp"
 
 namespace cles {
@@ -654,6 +656,113 @@ __global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs<T> a, int ntx, 
   a.nut[n] = !in || nv > 0.1 ? 0.0 : cs * a.fil2[n] * double(a.sabs[n]);
 }
 
+// The same two 27-point means as plane marches (used for the dynamic model
+// without homogeneous directions; the node tiles above serve the others): a workgroup owns a 16 x 16 column of nodes
+// (boundary layers included) and walks a chunk of kc planes; each plane's two
+// node values with a one-node halo are staged in LDS once, every thread forms
+// the 9-point sums of its node over the plane, and the node's 27-point sum is
+// P(k-1) + P(k) + P(k+1) from the sums of three planes kept in registers.
+//   PART 0: Cs from the means of LM and MM (les_cs_kernel);
+//   PART 1: the volume-weighted mean of Cs, then nu_t (les_smooth_kernel).
+template <typename T, int PART>
+__global__ __launch_bounds__(256) void les_mean_kernel(LesArgs<T> a, int ntx, int nty, int kc, int ntiles, int per_xcd) {
+  using V = typename std::conditional<PART == 0, T, double>::type;
+  __shared__ V B[2][FNB];
+  const Grid &g = a.g;
+  const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);  // contiguous tiles per XCD
+  if (tile >= ntiles) return;
+  const int i0 = (tile % ntx) * FTX, j0 = ((tile / ntx) % nty) * FTY;
+  const int k0 = (tile / (ntx * nty)) * kc;
+  const int k1 = k0 + kc - 1 < g.mz - 1 ? k0 + kc - 1 : g.mz - 1;  // last plane of the chunk
+  const int t = threadIdx.x;
+  const int lx = t % FTX, ly = t / FTX;
+  const int i = i0 + lx, j = j0 + ly;
+  const bool col = i < g.mx && j < g.my;
+  const int c0 = (ly + 1) * FBX + (lx + 1);  // slot of the column's node
+  V Pm[2] = {V(0), V(0)}, P0[2] = {V(0), V(0)};
+  for (int kk = k0 - 1; kk <= k1 + 1; kk++) {
+    const bool st = kk >= 0 && kk <= g.mz - 1;  // uniform
+    if (st) {
+      for (int w = t; w < FNB; w += 256) {
+        const int ii = i0 - 1 + w % FBX, jj = j0 - 1 + w / FBX;
+        if (ii < 0 || jj < 0 || ii > g.mx - 1 || jj > g.my - 1) continue;
+        const long long m = nid(g, kk, jj, ii);
+        if (PART == 0) {
+          const bool in = interior(g, kk, jj, ii);
+          B[0][w] = in ? a.lm[m] : T(0);  // LM is zero off the loop's nodes
+          B[1][w] = in ? a.mm[m] : T(0);
+        } else {
+          B[0][w] = a.ivol[m];  // 1. / aj
+          B[1][w] = lcs_at(a, kk, jj, ii);
+        }
+      }
+    }
+    __syncthreads();
+    V P1[2] = {V(0), V(0)};
+    if (st && col && i >= 1 && j >= 1 && i <= g.mx - 2 && j <= g.my - 2) {
+#pragma unroll
+      for (int b = -1; b <= 1; b++)
+#pragma unroll
+        for (int aa = -1; aa <= 1; aa++) {
+          const int o = c0 + b * FBX + aa;
+          if (PART == 0) {
+            P1[0] += B[0][o];
+            P1[1] += B[1][o];
+          } else {
+            const double w = B[0][o];
+            P1[0] += w * B[1][o];
+            P1[1] += w;
+          }
+        }
+    }
+    __syncthreads();  // the next plane overwrites the buffer
+    const int k = kk - 1;
+    if (col && k >= k0 && k <= k1) {
+      const long long n = nid(g, k, j, i);
+      const bool in = interior(g, k, j, i);
+      const double nv = nvat(a, n);
+      if (PART == 0) {
+        double cs = a.cs[n];
+        if (in) {
+          if (nv > 1.1) {
+            cs = 0;
+          } else if (a.les == 1) {
+            cs = 0.01;
+          } else {
+            const T LM_avg = ((Pm[0] + P0[0]) + P1[0]) * T(1.0 / 27.0);
+            const T MM_avg = ((Pm[1] + P0[1]) + P1[1]) * T(1.0 / 27.0);
+            const double C = double(LM_avg) / (double(MM_avg) + LES_EPS);
+            cs = C < 0.0001 ? 0.0001 : C;
+          }
+        }
+        if (nv > 1.1 && nv < 7.) {
+          cs = 0.0;
+        } else {
+          if (nv > 0.1 && nv < 1.1) cs = 0.01 < cs ? cs : 0.01;
+          double tt = cs < 0.000 ? 0.000 : cs;
+          cs = tt < a.max_cs ? tt : a.max_cs;
+        }
+        a.cs[n] = cs;
+      } else {
+        double cs;
+        if (!in || !(nv < 0.1)) {
+          cs = a.cs[n];
+        } else {
+          const double vs = (Pm[0] + P0[0]) + P1[0], ws = (Pm[1] + P0[1]) + P1[1];
+          cs = vs * rcp64(ws);
+        }
+        a.cs_out[n] = cs;
+        a.nut[n] = !in || nv > 0.1 ? 0.0 : cs * a.fil2[n] * double(a.sabs[n]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      Pm[q] = P0[q];
+      P0[q] = P1[q];
+    }
+  }
+}
+
 // nu_t = Cs Delta^2 |S| (les.c:1536-1573) on the paths without the gradient
 // loop (constant Cs, Cs = 0): lUcat is the velocity of the last CalcUCat
 template <typename T>
@@ -753,10 +862,24 @@ int launch_les_t(cles_ctx *c, hipStream_t stream, const double *const *qs_cell) 
   {
     const int ntx = (c->g.mx + MTX - 1) / MTX, nty = (c->g.my + MTY - 1) / MTY, ntz = (c->g.mz + MTZ - 1) / MTZ;
     const int ntiles = ntx * nty * ntz, pxm = (ntiles + 7) / 8;
-    hipLaunchKernelGGL(les_cs_kernel<T>, dim3(8 * pxm), dim3(256), 0, stream, a, ntx, nty, ntiles, pxm);
-    CLES_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(les_smooth_kernel<T>, dim3(8 * pxm), dim3(256), 0, stream, a, ntx, nty, ntiles, pxm);
-    CLES_HIP(c, hipGetLastError());
+    if (a.ih || a.jh || a.kh || a.les == 3) {
+      hipLaunchKernelGGL(les_cs_kernel<T>, dim3(8 * pxm), dim3(256), 0, stream, a, ntx, nty, ntiles, pxm);
+      CLES_HIP(c, hipGetLastError());
+      hipLaunchKernelGGL(les_smooth_kernel<T>, dim3(8 * pxm), dim3(256), 0, stream, a, ntx, nty, ntiles, pxm);
+      CLES_HIP(c, hipGetLastError());
+    } else {
+      const int mtx = (c->g.mx + FTX - 1) / FTX, mty = (c->g.my + FTY - 1) / FTY;
+      // 16-plane chunks (at 256^3: 16 planes 2.35 ms of LES model, 8 2.36, 32
+      // 2.43, the node tiles below 2.39), fewer where a launch would hold
+      // fewer than 1024 workgroups
+      int kc = 16;
+      while (kc > 4 && mtx * mty * ((c->g.mz + kc - 1) / kc) < 1024) kc /= 2;
+      const int nt = mtx * mty * ((c->g.mz + kc - 1) / kc), px = (nt + 7) / 8;
+      hipLaunchKernelGGL((les_mean_kernel<T, 0>), dim3(8 * px), dim3(256), 0, stream, a, mtx, mty, kc, nt, px);
+      CLES_HIP(c, hipGetLastError());
+      hipLaunchKernelGGL((les_mean_kernel<T, 1>), dim3(8 * px), dim3(256), 0, stream, a, mtx, mty, kc, nt, px);
+      CLES_HIP(c, hipGetLastError());
+    }
   }
   double *t = c->cs;
   c->cs = c->cs_alt;
