@@ -223,7 +223,7 @@ size_t ctx_footprint(const cles_params *p) {
     if (p->calc_sigma) planes += 6;  // shock sensor and its smoothing buffer
     if (p->les != 1 && p->les != 10) {
       planes += 1;  // 1/aj
-      extra += les_field_doubles(p, 7 * np) + 3 * les_field_doubles(p, np);  // strain rates + |S|, LM, MM, |S| of nu_t
+      extra += les_field_doubles(p, 8 * np) + 2 * les_field_doubles(p, np);  // strain rates, |S| and |S| of nu_t, LM, MM
       if (p->i_homo_filter) extra += homo_doubles(p);
     }
   }
@@ -311,10 +311,9 @@ cles_ctx *cles_create(const cles_params *p, const double *coor, int *err) {
     TRY(dalloc(c, &c->nut, np));
     if (p->les != 1 && p->les != 10) {
       // the model's own fields in the precision of the form (les_field_doubles)
-      TRY(dalloc(c, &c->grad, les_field_doubles(p, 7 * np)));
+      TRY(dalloc(c, &c->grad, les_field_doubles(p, 8 * np)));
       TRY(dalloc(c, &c->lm, les_field_doubles(p, np)));
       TRY(dalloc(c, &c->mm, les_field_doubles(p, np)));
-      TRY(dalloc(c, &c->sabs, les_field_doubles(p, np)));
       TRY(dalloc(c, &c->ivol, np));
       if (p->i_homo_filter) TRY(dalloc(c, &c->homo, homo_doubles(p)));
     }
@@ -344,7 +343,7 @@ void cles_destroy(cles_ctx *c) {
   member_leave(c);  // waits for its batch, which then rebuilds without it
   double *bufs[] = {c->coor, c->csi, c->eta, c->zet, c->aj, c->nvert, c->Q[0], c->Q[1], c->Q[2], c->P[0],
                     c->P[1], c->P[2], c->rhs, c->xfer, c->cs, c->cs_alt, c->nut, c->grad, c->lm, c->mm,
-                    c->fil2, c->qsnap, c->tfil2, c->ivol, c->sabs, c->avg, c->sgm, c->sgm_tmp, c->homo, c->mpack};
+                    c->fil2, c->qsnap, c->tfil2, c->ivol, c->avg, c->sgm, c->sgm_tmp, c->homo, c->mpack};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   ibm_free(c);

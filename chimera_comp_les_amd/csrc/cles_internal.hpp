@@ -274,9 +274,9 @@ struct cles_ctx {
   double *cs, *cs_alt, *nut;
   // fields of the dynamic model in the precision of the form (double, or
   // float for FP32-bulk; the pointers then address float planes of np)
-  double *grad;    // 7 planes: strain-rate components S11 S12 S13 S22 S23 S33, |S|
+  double *grad;    // the gradient kernel's 8 values per node in blocks of B = 8 (double) or 16 (float) nodes
+                   // ([np/B][4][B][2] of the form's type): pairs (S11, S12), (S13, S22), (S23, S33), (|S|, |S| of nu_t)
   double *lm, *mm;
-  double *sabs;       // |S| of the velocity after the gradient loop, for nu_t
   double *homo;           // -i_homo_filter: line, plane and global sums of LM and MM
   double *sgm, *sgm_tmp;  // -calc_sigma: 3 planes each, the shock sensor (sgm is the one the fluxes read)
   double *ivol;       // 1/aj, the smoothing weights

@@ -46,8 +46,11 @@ struct LesArgs {
   const double *csi, *eta, *zet, *aj, *nv;
   const double *mpack;  // the nine metrics and aj in blocks of 8 nodes (as in the stage kernel)
   int has_ib;
-  T *grad;  // 7 planes: S11 S12 S13 S22 S23 S33 (no trace removal) and |S|
-  T *sabs;  // |S| of the velocity lUcat holds after the gradient loop (nu_t)
+  // the gradient kernel's eight values per node in blocks of 8 nodes in
+  // double, 16 in float ([np/B][4][B][2]: pairs (S11, S12), (S13, S22),
+  // (S23, S33) (no trace removal), (|S|, |S| of the velocity lUcat holds
+  // after the loop, for nu_t)), so that readers take two values per load
+  T *grad;
   T *lm, *mm;
   const double *ivol;   // 1/aj
   double *cs, *cs_out, *nut;
@@ -136,6 +139,20 @@ template <typename T> __device__ __forceinline__ T sabs_of(const T g[9]) {
                         Szy * Szy + Szz * Szz));
 }
 
+// pair p of node n in the gradient pack
+template <typename T> struct Pair2;
+template <> struct Pair2<double> { using type = double2; };
+template <> struct Pair2<float> { using type = float2; };
+// (blocks of GB nodes: one pair of a block fills a 128-byte line)
+template <typename T> __device__ __forceinline__ typename Pair2<T>::type *gpair(T *pack, long long n, int p) {
+  constexpr int GB = 64 / sizeof(T);
+  return reinterpret_cast<typename Pair2<T>::type *>(pack + ((n / GB) * 4 + p) * (2 * GB) + (n % GB) * 2);
+}
+template <typename T> __device__ __forceinline__ T gval(const T *pack, long long n, int c) {
+  constexpr int GB = 64 / sizeof(T);
+  return pack[((n / GB) * 4 + (c >> 1)) * (2 * GB) + (n % GB) * 2 + (c & 1)];
+}
+
 // Node-range kernels: each XCD (blockIdx % 8) walks a contiguous range of
 // blocks, so the k +- 1 neighbours of its nodes stay in its own L2.
 __device__ __forceinline__ long long xcd_node(int per_xcd) {
@@ -155,10 +172,10 @@ __global__ __launch_bounds__(256) void les_grad_kernel(LesArgs<T> a, int per_xcd
   if (n >= (long long)g.mx * g.my * g.mz) return;
   const int i = (int)(n % g.mx), j = (int)((n / g.mx) % g.my), k = (int)(n / g.sz);
   const long long np = g.np;
+  using P2 = typename Pair2<T>::type;
   if (!in_loop(a, k, j, i)) {
 #pragma unroll
-    for (int c = 0; c < 7; c++) a.grad[n + c * np] = T(0);
-    a.sabs[n] = T(0);
+    for (int p = 0; p < 4; p++) *gpair(a.grad, n, p) = P2{T(0), T(0)};
     return;
   }
   T up_i[3], up_j[3], up_k[3], um_i[3], um_j[3], um_k[3];
@@ -170,19 +187,16 @@ __global__ __launch_bounds__(256) void les_grad_kernel(LesArgs<T> a, int per_xcd
   ua_at(a, k - 1, j, i, um_k);
   T gr[9];
   grad9(a, n, up_i, um_i, up_j, um_j, up_k, um_k, gr);
-  a.grad[n] = T(0.5) * (gr[0] + gr[0]);
-  a.grad[n + np] = T(0.5) * (gr[1] + gr[3]);
-  a.grad[n + 2 * np] = T(0.5) * (gr[2] + gr[6]);
-  a.grad[n + 3 * np] = T(0.5) * (gr[4] + gr[4]);
-  a.grad[n + 4 * np] = T(0.5) * (gr[5] + gr[7]);
-  a.grad[n + 5 * np] = T(0.5) * (gr[8] + gr[8]);
-  a.grad[n + 6 * np] = sabs_of(gr);
+  *gpair(a.grad, n, 0) = P2{T(0.5) * (gr[0] + gr[0]), T(0.5) * (gr[1] + gr[3])};
+  *gpair(a.grad, n, 1) = P2{T(0.5) * (gr[2] + gr[6]), T(0.5) * (gr[4] + gr[4])};
+  *gpair(a.grad, n, 2) = P2{T(0.5) * (gr[5] + gr[7]), T(0.5) * (gr[8] + gr[8])};
+  const T s_loop = sabs_of(gr);
   // nu_t: the +1 neighbours as lUcat holds them after the loop
   if (in_loop(a, k, j, i + 1)) vel(a.q, n + 1, np, up_i);
   if (in_loop(a, k, j + 1, i)) vel(a.q, n + g.sy, np, up_j);
   if (in_loop(a, k + 1, j, i)) vel(a.q, n + g.sz, np, up_k);
   grad9(a, n, up_i, um_i, up_j, um_j, up_k, um_k, gr);
-  a.sabs[n] = sabs_of(gr);
+  *gpair(a.grad, n, 3) = P2{s_loop, sabs_of(gr)};
 }
 
 // gradient component as seen by the test filter: the local vectors lSx..lSz
@@ -273,13 +287,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
         B[6][w] = u[2];
       }
       if (DM) {
-        long long ms = m;
-        T S = T(0);
-        if (interior(g, kk, jj, ii)) S = a.grad[m + 6 * np];
-        else ms = seam_image(g, kk, jj, ii);
-#pragma unroll
-        for (int c = 0; c < 6; c++) B[QS + c][w] = ms >= 0 ? a.grad[ms + c * np] : T(0);
-        B[QS + 6][w] = S;
+        using P2 = typename Pair2<T>::type;
+        if (interior(g, kk, jj, ii)) {
+          const P2 x0 = *gpair(a.grad, m, 0), x1 = *gpair(a.grad, m, 1), x2 = *gpair(a.grad, m, 2), x3 = *gpair(a.grad, m, 3);
+          B[QS][w] = x0.x;
+          B[QS + 1][w] = x0.y;
+          B[QS + 2][w] = x1.x;
+          B[QS + 3][w] = x1.y;
+          B[QS + 4][w] = x2.x;
+          B[QS + 5][w] = x2.y;
+          B[QS + 6][w] = x3.x;
+        } else {
+          const long long ms = seam_image(g, kk, jj, ii);
+          P2 x0{T(0), T(0)}, x1{T(0), T(0)}, x2{T(0), T(0)};
+          if (ms >= 0) {
+            x0 = *gpair(a.grad, ms, 0);
+            x1 = *gpair(a.grad, ms, 1);
+            x2 = *gpair(a.grad, ms, 2);
+          }
+          B[QS][w] = x0.x;
+          B[QS + 1][w] = x0.y;
+          B[QS + 2][w] = x1.x;
+          B[QS + 3][w] = x1.y;
+          B[QS + 4][w] = x2.x;
+          B[QS + 5][w] = x2.y;
+          B[QS + 6][w] = T(0);  // |S| without a periodic copy
+        }
       }
     }
     const bool act1 = col && kk >= 1 && kk <= g.mz - 2 && !(nvat(a, nid(g, kk, j, i)) > 1.1);
@@ -289,8 +322,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
     T Sx1[6];
     if (!DM) {
       const long long n1 = act1 ? nid(g, kk, j, i) : 0;
-#pragma unroll
-      for (int c = 0; c < 6; c++) Sx1[c] = act1 ? a.grad[n1 + c * np] : T(0);
+      using P2 = typename Pair2<T>::type;
+      P2 x0{T(0), T(0)}, x1{T(0), T(0)}, x2{T(0), T(0)};
+      if (act1) {
+        x0 = *gpair(a.grad, n1, 0);
+        x1 = *gpair(a.grad, n1, 1);
+        x2 = *gpair(a.grad, n1, 2);
+      }
+      Sx1[0] = x0.x;
+      Sx1[1] = x0.y;
+      Sx1[2] = x1.x;
+      Sx1[3] = x1.y;
+      Sx1[4] = x2.x;
+      Sx1[5] = x2.y;
     }
     __syncthreads();
     // ---- the plane sums of this column's node
@@ -653,7 +697,7 @@ __global__ __launch_bounds__(256) void les_smooth_kernel(LesArgs<T> a, int ntx, 
     cs = vs * rcp64(ws);
   }
   a.cs_out[n] = cs;
-  a.nut[n] = !in || nv > 0.1 ? 0.0 : cs * a.fil2[n] * double(a.sabs[n]);
+  a.nut[n] = !in || nv > 0.1 ? 0.0 : cs * a.fil2[n] * double(gval(a.grad, n, 7));
 }
 
 // The same two 27-point means as plane marches (used for the dynamic model
@@ -752,7 +796,7 @@ __global__ __launch_bounds__(256) void les_mean_kernel(LesArgs<T> a, int ntx, in
           cs = vs * rcp64(ws);
         }
         a.cs_out[n] = cs;
-        a.nut[n] = !in || nv > 0.1 ? 0.0 : cs * a.fil2[n] * double(a.sabs[n]);
+        a.nut[n] = !in || nv > 0.1 ? 0.0 : cs * a.fil2[n] * double(gval(a.grad, n, 7));
       }
     }
 #pragma unroll
@@ -803,7 +847,7 @@ int launch_les_t(cles_ctx *c, hipStream_t stream, const double *const *qs_cell) 
   a.csi = c->csi; a.eta = c->eta; a.zet = c->zet; a.aj = c->aj; a.nv = c->nvert;
   a.mpack = c->mpack;
   a.has_ib = c->has_ib;
-  a.grad = (T *)c->grad; a.lm = (T *)c->lm; a.mm = (T *)c->mm; a.sabs = (T *)c->sabs;
+  a.grad = (T *)c->grad; a.lm = (T *)c->lm; a.mm = (T *)c->mm;
   a.ivol = c->ivol; a.fil2 = c->fil2; a.tfil2 = c->tfil2;
   a.cs = c->cs; a.cs_out = c->cs_alt; a.nut = c->nut;
   a.les = p.les; a.testfilter_ik = p.testfilter_ik; a.max_cs = p.max_cs; a.twoD = p.twoD;
