@@ -139,11 +139,11 @@ template <typename T> __device__ __forceinline__ T sabs_of(const T g[9]) {
                         Szy * Szy + Szz * Szz));
 }
 
-// pair p of node n in the gradient pack
+// pair p (values 2p, 2p + 1) of node n in the gradient pack, and value c:
+// blocks of GB nodes, so that one pair of a block fills a 128-byte line
 template <typename T> struct Pair2;
 template <> struct Pair2<double> { using type = double2; };
 template <> struct Pair2<float> { using type = float2; };
-// (blocks of GB nodes: one pair of a block fills a 128-byte line)
 template <typename T> __device__ __forceinline__ typename Pair2<T>::type *gpair(T *pack, long long n, int p) {
   constexpr int GB = 64 / sizeof(T);
   return reinterpret_cast<typename Pair2<T>::type *>(pack + ((n / GB) * 4 + p) * (2 * GB) + (n % GB) * 2);
