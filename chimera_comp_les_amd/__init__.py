@@ -72,8 +72,13 @@ EXPORTS = [
     "cles_batch_last_error", "cles_set_tile", "cles_batch_launch", "cles_batch_wait",
     "cles_set_state_qp", "cles_write_fields", "cles_read_fields", "cles_petsc_vec_write", "cles_petsc_vec_read",
     "cles_get_averages", "cles_ibm_forces", "cles_write_vtk", "cles_ibm_forces_noalloc",
-    "cles_batch_set_mode", "cles_batch_set_spin_limit", "cles_batch_set_graph",
+    "cles_batch_set_mode", "cles_batch_set_spin_limit", "cles_batch_set_graph", "cles_write_vtk_opts",
 ]
+
+class CVtkOpts(C.Structure):
+    """cles_vtk_opts of include/cles.h: VTKOut's -skip, -theta, -Pres, -airfoil."""
+    _fields_ = [("skip", C.c_int), ("theta", C.c_double), ("pres", C.c_int), ("airfoil", C.c_int)]
+
 
 IBM_INFO_FIELDS = ("ni", "nj", "nk", "cell", "imode", "i1", "j1", "k1", "i2", "j2", "k2", "i3", "j3", "k3")
 
@@ -161,6 +166,8 @@ def load_library(path=LIB_PATH):
     L.cles_ibm_forces_noalloc.argtypes = [vp, dp, dp]
     L.cles_write_fields.argtypes = [vp, C.c_char_p, C.c_int]
     L.cles_write_vtk.argtypes = [vp, C.c_char_p, C.c_int]
+    L.cles_write_vtk_opts.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(CVtkOpts), dp, C.c_char_p, C.c_size_t,
+                                      C.POINTER(C.c_size_t)]
     L.cles_read_fields.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
     L.cles_petsc_vec_write.argtypes = [C.c_char_p, dp, C.c_longlong, C.c_int]
     L.cles_petsc_vec_read.argtypes = [C.c_char_p, dp, C.c_longlong]
@@ -252,12 +259,36 @@ class Solver:
 
 
     def write_vtk(self, directory=None, block=0):
-        """VTKOut (data.c:385-790): Result%2.2d_%6.6d.vts of the current state
+        """VTKOut (data.c:385-770): Result%2.2d_%6.6d.vts of the current state
         in directory (default: the working directory); returns its path."""
         d = None if directory is None else os.fsencode(directory)
         self._check(self.L.cles_write_vtk(self.h, d, int(block)))
         name = "Result%2.2d_%6.6d.vts" % (int(block), self.ti)
         return name if directory is None else os.path.join(os.fspath(directory), name)
+
+    def vtk_out(self, directory=None, block=0, skip=1, theta=0.0, pres=0, airfoil=0):
+        """VTKOut with its options (data.c:385-855).  Without pres: writes the
+        file of write_vtk with every skip-th x node (block 2 rotated by theta)
+        and returns (path, tke), tke the sum the reference prints as "TKE".
+        With pres: writes nothing and returns (None, text), text the surface
+        pressure lines the reference prints (block 2 along x node 20, airfoil
+        along x node 8)."""
+        d = None if directory is None else os.fsencode(directory)
+        o = CVtkOpts(int(skip), float(theta), int(pres), int(airfoil))
+        tke = C.c_double(0.0)
+        used = C.c_size_t(0)
+        cap = 1 << 16
+        while True:
+            buf = C.create_string_buffer(cap)
+            self._check(self.L.cles_write_vtk_opts(self.h, d, int(block), C.byref(o), C.byref(tke), buf, cap,
+                                                   C.byref(used)))
+            if used.value < cap:
+                break
+            cap = used.value + 1
+        if pres:
+            return None, buf.value.decode()
+        name = "Result%2.2d_%6.6d.vts" % (int(block), self.ti)
+        return (name if directory is None else os.path.join(os.fspath(directory), name)), tke.value
 
     def read_fields(self, ti, directory=None, block=0):
         """Q_P_Binary_Input + RestartApplication from the files of step ti."""

@@ -193,14 +193,31 @@ struct VtkArray {
 }  // namespace
 }  // namespace cles
 
-extern "C" int cles_write_vtk(cles_ctx *c, const char *dir, int block) {
+// VTKOut (data.c:385-855) with its options.  The extents are those the
+// reference writes: -twoD keeps one x/y/z node, -skip s > 1 keeps mx / s x
+// nodes (node i from x node i s); the fields under -twoD 1 come from x node 1
+// (Sigma from i + 1), the coordinates from node i (or i s).  Block 2's
+// positions and velocities are rotated by theta about z (data.c:544-548,
+// 678-681).  With pres the reference writes no file and prints the pressure
+// of the first fluid nodes next to a body along the x node 20 (block 2) and
+// 8 (airfoil) lines (data.c:772-844), which go to text here.  The rotation
+// and the TKE sum are evaluated without fused multiply-adds, as the
+// reference's C compiles on x86-64.
+static int vtk_out(cles_ctx *c, const char *dir, int block, const cles_vtk_opts *o, double *tke, char *text,
+                   size_t text_len, size_t *text_used) {
+#pragma clang fp contract(off)
   if (!c) return CLES_EINVAL;
   if (int g_ = member_guard(c, false)) return g_;
   if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
+  const int skip = o ? o->skip : 1;
+  const double theta = o ? o->theta : 0.0;
+  const int pres = o ? o->pres : 0, airfoil = o ? o->airfoil : 0;
   const int MX = c->g.mx, MY = c->g.my, MZ = c->g.mz;
   const long long N = (long long)MX * MY * MZ;
   const int twoD = c->p.twoD;
-  const int mx = twoD == 1 ? 1 : MX, my = twoD == 2 ? 1 : MY, mz = twoD == 3 ? 1 : MZ;
+  int mx = twoD == 1 ? 1 : MX;
+  const int my = twoD == 2 ? 1 : MY, mz = twoD == 3 ? 1 : MZ;
+  if (skip > 1) mx = mx / skip;  // node (mx - 1) skip <= MX - skip stays on the grid
   const long long nn = (long long)mx * my * mz;
   if (3 * nn * 8 > INT32_MAX)
     return set_error(c, CLES_EINVAL, "VTK output: an array of %lld bytes does not fit the 4-byte size header", 3 * nn * 8);
@@ -209,6 +226,46 @@ extern "C" int cles_write_vtk(cles_ctx *c, const char *dir, int block) {
   if (r) return r;
   if ((r = cles_get_nvert(c, nv.data()))) return r;
   if ((r = get_coor(c, xyz.data()))) return r;
+  auto at = [&](int k, int j, int i) { return ((long long)k * MY + j) * MX + i; };
+  if (pres) {
+    // data.c:774-842: " PL \n " then, per line (block 2's, then airfoil's),
+    // the nodes whose next node (k + 1, or j + 1 for airfoil) is solid,
+    // " PU \n " and those whose previous one is; the loops run over nodes
+    // 1 .. n - 3 of the written extents
+    std::string txt = " PL \n ";
+    char line[256];
+    auto out_line = [&](const char *fmt, double a0, double a1, double a2, double a3, int n) {
+      const int len = n == 4 ? snprintf(line, sizeof(line), fmt, a0, a1, a2, a3) : snprintf(line, sizeof(line), fmt, a0, a1, a2);
+      txt.append(line, (size_t)len);
+    };
+    for (int part = 0; part < 2; part++) {
+      if (part == 0 ? block != 2 : !airfoil) continue;
+      const int i = part == 0 ? 20 : 8;
+      if (i > MX - 1) return set_error(c, CLES_EINVAL, "pressure lines: x node %d outside the grid", i);
+      for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) txt += " PU \n ";
+        for (int a0 = 1; a0 < (part == 0 ? my : mz) - 2; a0++)
+          for (int b0 = 1; b0 < (part == 0 ? mz : my) - 2; b0++) {
+            const int j = part == 0 ? a0 : b0, k = part == 0 ? b0 : a0;
+            const long long n = at(k, j, i);
+            const long long nb = part == 0 ? at(pass == 0 ? k + 1 : k - 1, j, i) : at(k, pass == 0 ? j + 1 : j - 1, i);
+            if (!(nv[n] == 0.0 && nv[nb] > 0.0)) continue;
+            if (pass == 0)
+              out_line("%f %f  %f  %f\n ", xyz[3 * n], xyz[3 * n + 2], xyz[3 * n + 1], p[n], 4);
+            else
+              out_line("%f  %f  %f   \n ", xyz[3 * n + 2], xyz[3 * n + 1], p[n], 0.0, 3);
+          }
+      }
+    }
+    if (text_used) *text_used = txt.size();
+    if (text && text_len) {
+      const size_t m = txt.size() < text_len - 1 ? txt.size() : text_len - 1;
+      memcpy(text, txt.data(), m);
+      text[m] = '\0';
+    }
+    if (tke) *tke = 0.0;
+    return CLES_OK;
+  }
   const bool les = c->p.les != 0;
   if (les) {
     nut.resize((size_t)N);
@@ -253,35 +310,58 @@ extern "C" int cles_write_vtk(cles_ctx *c, const char *dir, int block) {
     const int bytes = (int)(nc * nn * 8);
     ok = ok && fwrite(&bytes, sizeof(int), 1, f) == 1 && fwrite(out.data(), 8, (size_t)(nc * nn), f) == (size_t)(nc * nn);
   };
-  // source node of output node (k, j, i): the fields read x node 1 under
-  // twoD 1 (Sigma: i + 1 = 1), the coordinates x node i
-  auto src = [&](int k, int j, int i, bool field) {
-    const int ii = field && twoD == 1 ? 1 : i;
-    return ((long long)k * MY + j) * MX + ii;
+  // source x node of output x node i: the coordinates i (or i skip), the
+  // fields i skip, else 1 under twoD 1, else i; Sigma i skip, but i + 1
+  // under twoD 1
+  auto src = [&](int k, int j, int i, int kind) {
+    int ii = i;
+    if (kind == 0) ii = skip > 1 ? i * skip : i;
+    else if (kind == 1) ii = skip > 1 ? i * skip : twoD == 1 ? 1 : i;
+    else ii = twoD == 1 ? i + 1 : skip > 1 ? i * skip : i;
+    return at(k, j, ii);
   };
+  const bool rot = block == 2;
+  const double ct = cos(theta), st = sin(theta);
   for (int k = 0; k < mz; k++)  // Position
     for (int j = 0; j < my; j++)
       for (int i = 0; i < mx; i++) {
-        const long long o = ((long long)k * my + j) * mx + i, s = src(k, j, i, false);
-        for (int d = 0; d < 3; d++) out[3 * o + d] = xyz[3 * s + d];
+        const long long o = ((long long)k * my + j) * mx + i, s = src(k, j, i, 0);
+        double x = xyz[3 * s], y = xyz[3 * s + 1];
+        if (rot) {
+          x = xyz[3 * s] * ct - xyz[3 * s + 1] * st;
+          y = xyz[3 * s + 1] * ct + xyz[3 * s] * st;
+        }
+        out[3 * o] = x;
+        out[3 * o + 1] = y;
+        out[3 * o + 2] = xyz[3 * s + 2];
       }
   put(3);
+  double sum = 0.0;
+  const double cells = (double)((long long)mz * my * mx);
   for (size_t kind = 0; kind < arrs.size(); kind++) {  // nvert p rho E u [Sigma nu_T]
     for (int k = 0; k < mz; k++)
       for (int j = 0; j < my; j++)
         for (int i = 0; i < mx; i++) {
-          const long long o = ((long long)k * my + j) * mx + i, s = src(k, j, i, true);
+          const long long o = ((long long)k * my + j) * mx + i, s = src(k, j, i, kind == 5 ? 2 : 1);
           const double *qs = &q[5 * s];
           switch (kind) {
             case 0: out[o] = nv[s]; break;
             case 1: out[o] = p[s]; break;
             case 2: out[o] = qs[0]; break;
             case 3: out[o] = qs[4] / qs[0]; break;
-            case 4:
-              out[3 * o] = qs[1] / qs[0];
-              out[3 * o + 1] = qs[2] / qs[0];
-              out[3 * o + 2] = qs[3] / qs[0];
+            case 4: {
+              const double ux = qs[1] / qs[0], uy = qs[2] / qs[0], uz = qs[3] / qs[0];
+              double x = ux, y = uy;
+              if (rot) {
+                x = ux * ct - uy * st;
+                y = uy * ct + ux * st;
+              }
+              out[3 * o] = x;
+              out[3 * o + 1] = y;
+              out[3 * o + 2] = uz;
+              sum += (x * x + y * y * y + uz * uz) / cells;  // the reference's "TKE" (v cubed), data.c:688
               break;
+            }
             case 5:
               for (int d = 0; d < 3; d++) out[3 * o + d] = sig[3 * s + d];
               break;
@@ -293,5 +373,18 @@ extern "C" int cles_write_vtk(cles_ctx *c, const char *dir, int block) {
   fprintf(f, "\n </AppendedData>\n");
   fprintf(f, "</VTKFile>\n");
   ok = (fclose(f) == 0) && ok;
+  if (tke) *tke = sum;
+  if (text_used) *text_used = 0;
+  if (text && text_len) text[0] = '\0';
   return ok ? CLES_OK : set_error(c, CLES_EINVAL, "write error on %s", path);
+}
+
+extern "C" int cles_write_vtk(cles_ctx *c, const char *dir, int block) {
+  return vtk_out(c, dir, block, nullptr, nullptr, nullptr, 0, nullptr);
+}
+
+extern "C" int cles_write_vtk_opts(cles_ctx *c, const char *dir, int block, const cles_vtk_opts *o, double *tke,
+                                   char *text, size_t text_len, size_t *text_used) {
+  if (o && o->skip < 1) return c ? set_error(c, CLES_EINVAL, "skip must be at least 1") : CLES_EINVAL;
+  return vtk_out(c, dir, block, o, tke, text, text_len, text_used);
 }

@@ -186,6 +186,27 @@ int cles_write_fields(cles_ctx *c, const char *dir, int block);
  * after a 4-byte byte count (so each array stays under 2 GiB); -twoD takes
  * one x/y/z node as the reference's writer does. */
 int cles_write_vtk(cles_ctx *c, const char *dir, int block);
+/* VTKOut's options (data.c:423-429): -skip (keep every skip-th x node, at
+ * least 1), -theta (rotation of block 2's positions and velocities about z,
+ * radians), -Pres (no file: surface-pressure lines) and -airfoil (with
+ * -Pres: the airfoil's lines too; without it no effect). */
+typedef struct cles_vtk_opts {
+  int skip;
+  double theta;
+  int pres, airfoil;
+} cles_vtk_opts;
+/* VTKOut with its options (data.c:385-855).  Without pres: the file of
+ * cles_write_vtk with mx / skip x nodes (node i from x node i skip), and for
+ * block 2 positions and velocities rotated by theta; *tke (if not NULL) the
+ * "TKE" sum the reference prints, sum of (u^2 + v^3 + w^2) / nodes over the
+ * written nodes.  With pres: no file; the text the reference prints
+ * (" PL \n ", then for block 2 along x node 20 and then with airfoil along
+ * x node 8: the fluid nodes whose next node along z (block 2) or y (airfoil)
+ * is solid, " PU \n ", those whose previous one is) goes to text
+ * (NUL-terminated, at most text_len bytes with the NUL; *text_used its full
+ * length); CLES_EINVAL when that x node is not on the grid. */
+int cles_write_vtk_opts(cles_ctx *c, const char *dir, int block, const cles_vtk_opts *o, double *tke,
+                        char *text, size_t text_len, size_t *text_used);
 int cles_read_fields(cles_ctx *c, const char *dir, int ti, int block);
 /* One PETSc binary Vec file: class id 1211214 and length as big-endian
  * int32, then n big-endian doubles; block_size > 1 also writes <path>.info

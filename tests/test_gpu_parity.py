@@ -505,3 +505,109 @@ def test_vtk_output(case, tmp_path):
         w = np.ascontiguousarray(want[name]).reshape(-1)
         assert nc * mx * my * mz == vals.size == w.size, name
         assert np.array_equal(vals, w), name
+
+
+def test_vtk_out_skip_rotation_tke(tmp_path):
+    """VTKOut's -skip and -theta (data.c:423-429, 465-470, 530-760): skip 3
+    keeps mx / 3 x nodes, node i from x node 3 i, for the coordinates and
+    every field (Sigma included); block 2 rotates positions and velocities
+    about z by theta (cos and sin of the C library, products not fused) and
+    leaves the other arrays alone; the "TKE" sum (u^2 + v^3 + w^2) / nodes
+    runs in the writer's k, j, i order, so numpy's sequential cumsum of the
+    same terms matches it bitwise.  Block 0 ignores theta."""
+    import math
+    cfg, coor, Q0 = cases.tgv_case(N=22)
+    g = cl.Solver(cfg, coor)
+    g.set_state(perturbed(Q0, 5, amp=1e-2))
+    g.advance(1)
+    theta, skip = 0.3, 3
+    path2, tke2 = g.vtk_out(str(tmp_path), block=2, skip=skip, theta=theta)
+    path0, tke0 = g.vtk_out(str(tmp_path), block=0, theta=theta)
+    assert os.path.basename(path2) == "Result02_%06d.vts" % g.ti
+    Q, P = g.state()
+    nv = g.nvert()
+    _, nut, sig = g.les_fields()
+    g.close()
+    mz, my, mx = Q.shape[:3]
+    ct, st = math.cos(theta), math.sin(theta)
+    for path, tke, sk, rot in ((path2, tke2, skip, True), (path0, tke0, 1, False)):
+        _, ext, arr = _read_vts(path)
+        nx = mx // sk
+        assert ext == [0, nx - 1, 0, my - 1, 0, mz - 1]
+        sl = (slice(None), slice(None), slice(0, nx * sk, sk))
+        X, U = coor[sl].copy(), Q[sl][..., 1:4] / Q[sl][..., 0:1]
+        if rot:
+            X[..., 0], X[..., 1] = coor[sl][..., 0] * ct - coor[sl][..., 1] * st, \
+                coor[sl][..., 1] * ct + coor[sl][..., 0] * st
+            u, v = U[..., 0].copy(), U[..., 1].copy()
+            U[..., 0], U[..., 1] = u * ct - v * st, v * ct + u * st
+        want = {"Position": X, "nvert": nv[sl], "p": P[sl], "rho": Q[sl][..., 0],
+                "E": Q[sl][..., 4] / Q[sl][..., 0], "u": U, "Sigma": sig[sl], "nu_T": nut[sl]}
+        assert list(arr) == list(want)
+        for name, (nc, vals) in arr.items():
+            assert np.array_equal(vals, np.ascontiguousarray(want[name]).reshape(-1)), name
+        terms = (U[..., 0] * U[..., 0] + U[..., 1] * U[..., 1] * U[..., 1] + U[..., 2] * U[..., 2]) / float(nx * my * mz)
+        assert tke == np.cumsum(terms.reshape(-1))[-1]
+    assert tke2 != tke0
+
+
+def _pressure_lines(coor, P, nv, block, airfoil):
+    """The surface-pressure text of VTKOut under -Pres (data.c:774-842),
+    restated: " PL \\n ", block 2's lines along x node 20 (j outer, k inner,
+    next/previous node along z), airfoil's along x node 8 (k outer, j inner,
+    along y), each as the first-side nodes, " PU \\n ", the second-side ones."""
+    mz, my = P.shape[:2]
+    out = " PL \n "
+    for i, along_z, on in ((20, True, block == 2), (8, False, airfoil)):
+        if not on:
+            continue
+        for side in (1, -1):
+            if side == -1:
+                out += " PU \n "
+            outer, inner = (my, mz) if along_z else (mz, my)
+            for a in range(1, outer - 2):
+                for b in range(1, inner - 2):
+                    j, k = (a, b) if along_z else (b, a)
+                    nb = nv[k + side, j, i] if along_z else nv[k, j + side, i]
+                    if not (nv[k, j, i] == 0.0 and nb > 0.0):
+                        continue
+                    x, y, z = coor[k, j, i]
+                    if side == 1:
+                        out += "%f %f  %f  %f\n " % (x, z, y, P[k, j, i])
+                    else:
+                        out += "%f  %f  %f   \n " % (z, y, P[k, j, i])
+    return out
+
+
+def test_vtk_out_pressure_lines(tmp_path):
+    """VTKOut under -Pres (data.c:772-844): no file, and the text of the
+    first fluid nodes beside a solid block (z nodes 9-12, y nodes 7-11) for
+    block 2 (x node 20, along z), the airfoil (x node 8, along y), both, and
+    neither; -airfoil without -Pres writes the ordinary file; a grid without
+    x node 20 is refused for block 2."""
+    cfg, coor, Q0 = cases.tgv_case(N=22)
+    nv = np.zeros(Q0.shape[:3])
+    nv[9:13, 7:12, :] = 1.0
+    g = cl.Solver(cfg, coor, nvert=nv)
+    g.set_state(perturbed(Q0, 6, amp=1e-2))
+    Q, P = g.state()
+    nvg = g.nvert()
+    assert np.array_equal(nvg, nv)
+    for block, airfoil in ((2, 1), (2, 0), (0, 1), (0, 0)):
+        path, text = g.vtk_out(str(tmp_path), block=block, pres=1, airfoil=airfoil)
+        assert path is None
+        assert text == _pressure_lines(coor, P, nvg, block, airfoil), (block, airfoil)
+    assert os.listdir(tmp_path) == []
+    assert text == " PL \n "
+    assert g.vtk_out(str(tmp_path), block=2, pres=1, airfoil=1)[1].count("\n") == 3 + 5 + 5 + 4 + 4
+    path, _ = g.vtk_out(str(tmp_path), block=1, airfoil=1)
+    assert os.listdir(tmp_path) == [os.path.basename(path)]
+    with pytest.raises(cl.CLESError):
+        g.vtk_out(str(tmp_path), skip=0)
+    g.close()
+    cfg, coor, Q0 = cases.tgv_case(N=10)
+    with cl.Solver(cfg, coor) as g:
+        g.set_state(Q0)
+        with pytest.raises(cl.CLESError):
+            g.vtk_out(str(tmp_path), block=2, pres=1)
+        assert g.vtk_out(str(tmp_path), block=0, pres=1)[1] == " PL \n "
