@@ -73,6 +73,7 @@ EXPORTS = [
     "cles_set_state_qp", "cles_write_fields", "cles_read_fields", "cles_petsc_vec_write", "cles_petsc_vec_read",
     "cles_get_averages", "cles_ibm_forces", "cles_write_vtk", "cles_ibm_forces_noalloc",
     "cles_batch_set_mode", "cles_batch_set_spin_limit", "cles_batch_set_graph", "cles_write_vtk_opts",
+    "cles_q_criteria", "cles_write_q_vtk",
 ]
 
 class CVtkOpts(C.Structure):
@@ -169,6 +170,8 @@ def load_library(path=LIB_PATH):
     L.cles_write_vtk_opts.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(CVtkOpts), dp, C.c_char_p, C.c_size_t,
                                       C.POINTER(C.c_size_t)]
     L.cles_read_fields.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
+    L.cles_q_criteria.argtypes = [vp, dp, dp]
+    L.cles_write_q_vtk.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
     L.cles_petsc_vec_write.argtypes = [C.c_char_p, dp, C.c_longlong, C.c_int]
     L.cles_petsc_vec_read.argtypes = [C.c_char_p, dp, C.c_longlong]
     _LIB = L
@@ -289,6 +292,21 @@ class Solver:
             return None, buf.value.decode()
         name = "Result%2.2d_%6.6d.vts" % (int(block), self.ti)
         return (name if directory is None else os.path.join(os.fspath(directory), name)), tke.value
+
+    def q_criteria(self):
+        """QCriteria (data.c:4815-5053): (Q criterion, lambda2) of the current
+        state, the boundary nodes holding the pressure and 0."""
+        q = np.empty(self.shape)
+        l2 = np.empty(self.shape)
+        self._check(self.L.cles_q_criteria(self.h, _ptr(q), _ptr(l2)))
+        return q, l2
+
+    def write_q_vtk(self, directory=None, block=0, vc=0):
+        """VTKOutQ (data.c:1562-1711): QCriteria%2.2d_%5.5d.vtk; returns its path."""
+        d = None if directory is None else os.fsencode(directory)
+        self._check(self.L.cles_write_q_vtk(self.h, d, int(block), int(vc)))
+        name = "QCriteria%2.2d_%5.5d.vtk" % (int(block), self.ti)
+        return name if directory is None else os.path.join(os.fspath(directory), name)
 
     def read_fields(self, ti, directory=None, block=0):
         """Q_P_Binary_Input + RestartApplication from the files of step ti."""

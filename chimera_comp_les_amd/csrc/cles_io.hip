@@ -379,6 +379,72 @@ static int vtk_out(cles_ctx *c, const char *dir, int block, const cles_vtk_opts 
   return ok ? CLES_OK : set_error(c, CLES_EINVAL, "write error on %s", path);
 }
 
+// VTKOutQ (data.c:1562-1711): the Q criterion and lambda2 of QCriteria
+// (data.c:4815-5053, cles_q_criteria) and nvert in a legacy ASCII VTK
+// structured grid of (mx-1) x (my-1) x (mz-1) points, the nodes 0..m-2.  With
+// vc 0 each point's value is the mean of the 8 nodes of the cell whose
+// lowest corner it is (0.125 times their sum, in the reference's order);
+// with vc 1 the node values of nodes 1..m-2 as CELL_DATA.  Values and
+// coordinates are printed with %f, as the reference does.
+extern "C" int cles_write_q_vtk(cles_ctx *c, const char *dir, int block, int vc) {
+  if (!c) return CLES_EINVAL;
+  if (int g_ = member_guard(c, false)) return g_;
+  if (!c->have_state) return set_error(c, CLES_ESTATE, "no state uploaded");
+  const int mx = c->g.mx, my = c->g.my, mz = c->g.mz;
+  const long long N = (long long)mx * my * mz;
+  std::vector<double> qc((size_t)N), l2((size_t)N), nv((size_t)N), xyz((size_t)N * 3);
+  int r = cles_q_criteria(c, qc.data(), l2.data());
+  if (r) return r;
+  if ((r = cles_get_nvert(c, nv.data()))) return r;
+  if ((r = get_coor(c, xyz.data()))) return r;
+  char path[1024];
+  if (dir && dir[0])
+    snprintf(path, sizeof(path), "%s/QCriteria%2.2d_%5.5d.vtk", dir, block, c->ti);
+  else
+    snprintf(path, sizeof(path), "QCriteria%2.2d_%5.5d.vtk", block, c->ti);
+  FILE *f = fopen(path, "w");
+  if (!f) return set_error(c, CLES_EINVAL, "cannot create %s: %s", path, strerror(errno));
+  auto at = [&](int k, int j, int i) { return ((long long)k * my + j) * mx + i; };
+  fprintf(f, "# vtk DataFile Version 2.0\n");
+  fprintf(f, "Structured Grid\n");
+  fprintf(f, "ASCII\n");
+  fprintf(f, "DATASET STRUCTURED_GRID\n");
+  fprintf(f, "DIMENSIONS %d %d %d\n", mx - 1, my - 1, mz - 1);
+  fprintf(f, "POINTS  %d double\n", (mx - 1) * (my - 1) * (mz - 1));
+  for (int k = 0; k < mz - 1; k++)
+    for (int j = 0; j < my - 1; j++)
+      for (int i = 0; i < mx - 1; i++) {
+        const long long n = at(k, j, i);
+        fprintf(f, "%f %f %f\n", xyz[3 * n], xyz[3 * n + 1], xyz[3 * n + 2]);
+      }
+  if (vc)
+    fprintf(f, "CELL_DATA %d\n", (mx - 2) * (my - 2) * (mz - 2));
+  else
+    fprintf(f, "POINT_DATA %d\n", (mx - 1) * (my - 1) * (mz - 1));
+  const int lo = vc ? 1 : 0;
+  const std::vector<double> *fields[3] = {&qc, &l2, &nv};
+  const char *names[3] = {"q", "l2", "nvert"};
+  for (int fi = 0; fi < 3; fi++) {
+    const std::vector<double> &v = *fields[fi];
+    fprintf(f, "SCALARS %s double\n", names[fi]);
+    fprintf(f, "LOOKUP_TABLE default\n");
+    for (int k = lo; k < mz - 1; k++)
+      for (int j = lo; j < my - 1; j++)
+        for (int i = lo; i < mx - 1; i++) {
+          double x;
+          if (vc)
+            x = v[at(k, j, i)];
+          else
+            x = 0.125 * (v[at(k, j, i)] + v[at(k, j, i + 1)] + v[at(k, j + 1, i)] + v[at(k, j + 1, i + 1)] +
+                         v[at(k + 1, j, i)] + v[at(k + 1, j, i + 1)] + v[at(k + 1, j + 1, i)] +
+                         v[at(k + 1, j + 1, i + 1)]);
+          fprintf(f, "%f\n", x);
+        }
+  }
+  const bool ok = fclose(f) == 0;
+  return ok ? CLES_OK : set_error(c, CLES_EINVAL, "write error on %s", path);
+}
+
 extern "C" int cles_write_vtk(cles_ctx *c, const char *dir, int block) {
   return vtk_out(c, dir, block, nullptr, nullptr, nullptr, 0, nullptr);
 }

@@ -207,6 +207,18 @@ typedef struct cles_vtk_opts {
  * length); CLES_EINVAL when that x node is not on the grid. */
 int cles_write_vtk_opts(cles_ctx *c, const char *dir, int block, const cles_vtk_opts *o, double *tke,
                         char *text, size_t text_len, size_t *text_used);
+/* QCriteria (data.c:4815-5053) of the post-processor's -qcr branch on the
+ * current state: q (the Q criterion, (|W|^2 - |S|^2) / 2 of the velocity
+ * gradient) and lambda2 (the middle of the reference's Jacobi eigenvalue
+ * estimates of its S^2 + W^2 products), each mx*my*mz in the global layout.
+ * Nodes 1..m-2 of each direction are computed; on the boundary nodes q is
+ * the pressure (the reference writes Q into P there only) and lambda2 0. */
+int cles_q_criteria(cles_ctx *c, double *q, double *lambda2);
+/* VTKOutQ (data.c:1562-1711): dir/QCriteria%2.2d_%5.5d.vtk (block, step), a
+ * legacy ASCII structured grid of the nodes 0..m-2 with q, l2 and nvert:
+ * 8-node cell means as POINT_DATA (vc 0), or the values of nodes 1..m-2 as
+ * CELL_DATA (vc 1, the reference's -vc). */
+int cles_write_q_vtk(cles_ctx *c, const char *dir, int block, int vc);
 int cles_read_fields(cles_ctx *c, const char *dir, int ti, int block);
 /* One PETSc binary Vec file: class id 1211214 and length as big-endian
  * int32, then n big-endian doubles; block_size > 1 also writes <path>.info

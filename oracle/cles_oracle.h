@@ -167,6 +167,14 @@ void orc_les_gradients(orc_ctx *c, double *A9, double *sabs);
  * every later dynamic-model update (NULL to stop). */
 void orc_les_debug(orc_ctx *c, double *lm, double *mm);
 
+/* Post-processor (oracle_post.c).  jacobi + eigsrt (data.c:5057-5203) on a
+ * row-major 3x3 matrix: the eigenvalue estimates in descending order. */
+void orc_jacobi3(const double *a, double *d);
+/* QCriteria (data.c:4815-5053) on global-layout arrays: Q (5/node), face
+ * metrics (3/node), aj, nvert; qc enters holding P, both outputs 1/node. */
+void orc_q_criteria(int mx, int my, int mz, const double *Q, const double *csi, const double *eta,
+                    const double *zet, const double *aj, const double *nvert, double *qc, double *l2);
+
 #ifdef __cplusplus
 }
 #endif

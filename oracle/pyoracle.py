@@ -116,6 +116,8 @@ def lib():
         L.orc_pow15.argtypes = [C.c_double]
         L.orc_glibc_first_rand.restype = C.c_double
         L.orc_glibc_first_rand.argtypes = [C.c_int]
+        L.orc_jacobi3.argtypes = [dp, dp]
+        L.orc_q_criteria.argtypes = [C.c_int] * 3 + [dp] * 8
         _LIB = L
     return _LIB
 
@@ -325,3 +327,23 @@ def glibc_first_rand(seed):
 
 def pow15(x):
     return lib().orc_pow15(float(x))
+
+
+def jacobi3(a):
+    """jacobi + eigsrt (data.c:5057-5203) of a 3x3 matrix: the eigenvalue
+    estimates in descending order."""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(3, 3)
+    d = np.empty(3)
+    lib().orc_jacobi3(_ptr(a), _ptr(d))
+    return d
+
+
+def q_criteria(Q, P, csi, eta, zet, aj, nvert):
+    """QCriteria (data.c:4815-5053) on global-layout arrays: returns (Q
+    criterion with the pressure kept on boundary nodes, lambda2)."""
+    mz, my, mx = Q.shape[:3]
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (Q, csi, eta, zet, aj, nvert)]
+    qc = np.array(P, dtype=np.float64, order="C")
+    l2 = np.empty((mz, my, mx))
+    lib().orc_q_criteria(mx, my, mz, *[_ptr(x) for x in arrs], _ptr(qc), _ptr(l2))
+    return qc, l2

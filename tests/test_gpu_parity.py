@@ -611,3 +611,70 @@ def test_vtk_out_pressure_lines(tmp_path):
         with pytest.raises(cl.CLESError):
             g.vtk_out(str(tmp_path), block=2, pres=1)
         assert g.vtk_out(str(tmp_path), block=0, pres=1)[1] == " PL \n "
+
+
+@pytest.mark.parametrize("case", ["tgv_solid", "channel"])
+def test_q_criteria(case):
+    """QCriteria (data.c:4815-5053) on the device equals the oracle's
+    restatement bitwise, from the same state, nvert and device metrics: a
+    periodic Taylor-Green state with solid nodes (one-sided differences around
+    them; nvert 1 is not solid there) and a stepped channel on a stretched grid;
+    boundary nodes keep the pressure and a lambda2 of 0."""
+    if case == "tgv_solid":
+        cfg, coor, Q0 = cases.tgv_case(N=12)
+        Q0 = perturbed(Q0, 8, amp=5e-2)
+        nv = np.zeros(Q0.shape[:3])
+        nv[5:8, 4:9, 6] = 3.0
+        nv[3, 3, 3] = 1.0
+    else:
+        cfg, coor, Q0 = channel_case(bc=(7, 7, 1, 1, 45, 8), iso=0, les=0)
+        nv = None
+    g = cl.Solver(cfg, coor, nvert=nv)
+    g.set_state(Q0)
+    if nv is None:
+        g.advance(1)
+    q, l2 = g.q_criteria()
+    Q, P = g.state()
+    nvg = g.nvert()
+    met = g.metrics()
+    g.close()
+    qo, l2o = pyoracle.q_criteria(Q, P, *met, nvg)
+    assert np.array_equal(q, qo)
+    assert np.array_equal(l2, l2o)
+    inner = (slice(1, -1),) * 3
+    assert np.abs(q[inner]).max() > 0 and np.abs(l2[inner]).max() > 0
+    border = np.ones(q.shape, bool)
+    border[inner] = False
+    assert np.array_equal(q[border], P[border]) and np.all(l2[border] == 0.0)
+
+
+@pytest.mark.parametrize("vc", [0, 1])
+def test_q_vtk_output(vc, tmp_path):
+    """VTKOutQ (data.c:1562-1711): the legacy ASCII file, text equal to the
+    one printed here from the device's Q criterion, lambda2 and nvert: points
+    of the nodes 0..m-2, then q, l2 and nvert as 8-node cell means
+    (POINT_DATA, vc 0) or the values of nodes 1..m-2 (CELL_DATA, vc 1)."""
+    cfg, coor, Q0 = cases.tgv_case(N=6)
+    nv = np.zeros(Q0.shape[:3])
+    nv[2:4, 3, 2:5] = 3.0
+    g = cl.Solver(cfg, coor, nvert=nv)
+    g.set_state(perturbed(Q0, 9, amp=5e-2))
+    path = g.write_q_vtk(str(tmp_path), block=1, vc=vc)
+    assert os.path.basename(path) == "QCriteria01_%05d.vtk" % g.ti
+    q, l2 = g.q_criteria()
+    g.close()
+    mz, my, mx = q.shape
+    out = ["# vtk DataFile Version 2.0", "Structured Grid", "ASCII", "DATASET STRUCTURED_GRID",
+           "DIMENSIONS %d %d %d" % (mx - 1, my - 1, mz - 1), "POINTS  %d double" % ((mx - 1) * (my - 1) * (mz - 1))]
+    out += ["%f %f %f" % tuple(x) for x in coor[:-1, :-1, :-1].reshape(-1, 3)]
+    out.append("CELL_DATA %d" % ((mx - 2) * (my - 2) * (mz - 2)) if vc else
+               "POINT_DATA %d" % ((mx - 1) * (my - 1) * (mz - 1)))
+    for name, v in (("q", q), ("l2", l2), ("nvert", nv)):
+        out += ["SCALARS %s double" % name, "LOOKUP_TABLE default"]
+        if vc:
+            vals = v[1:-1, 1:-1, 1:-1]
+        else:
+            vals = 0.125 * (v[:-1, :-1, :-1] + v[:-1, :-1, 1:] + v[:-1, 1:, :-1] + v[:-1, 1:, 1:]
+                            + v[1:, :-1, :-1] + v[1:, :-1, 1:] + v[1:, 1:, :-1] + v[1:, 1:, 1:])
+        out += ["%f" % x for x in vals.reshape(-1)]
+    assert open(path).read() == "\n".join(out) + "\n"
