@@ -65,6 +65,8 @@ int main(int argc, char **argv) {
          offsetof(cles_params, averaging), offsetof(cles_params, k_homo_filter));
   printf("sizeof cles_ibm_info %zu offsetof pmin %zu cr3 %zu\n", sizeof(cles_ibm_info),
          offsetof(cles_ibm_info, pmin), offsetof(cles_ibm_info, cr3));
+  printf("sizeof cles_vtk_opts %zu offsetof theta %zu pres %zu airfoil %zu\n", sizeof(cles_vtk_opts),
+         offsetof(cles_vtk_opts, theta), offsetof(cles_vtk_opts, pres), offsetof(cles_vtk_opts, airfoil));
   printf("%s %d\n", fails ? "FAILED" : "OK", fails);
   return fails != 0;
 }

@@ -46,6 +46,10 @@ def test_c_caller_builds_and_runs(tmp_path):
     m = re.search(r"sizeof cles_ibm_info (\d+) offsetof pmin (\d+) cr3 (\d+)", out.stdout)
     assert int(m.group(1)) == ctypes.sizeof(cl.CIbmInfo)
     assert cl.CIbmInfo.pmin.offset == int(m.group(2)) and cl.CIbmInfo.cr3.offset == int(m.group(3))
+    m = re.search(r"sizeof cles_vtk_opts (\d+) offsetof theta (\d+) pres (\d+) airfoil (\d+)", out.stdout)
+    assert int(m.group(1)) == ctypes.sizeof(cl.CVtkOpts)
+    for name, off in zip(("theta", "pres", "airfoil"), m.groups()[1:]):
+        assert getattr(cl.CVtkOpts, name).offset == int(off), name
     # the file the C program wrote is the PETSc binary format the Python side reads
     x = cl.petsc_vec_read(str(tmp_path / "vec.dat"), 37)
     assert all(x[n] == (n - 18) * 0.375 + 1e-300 * n for n in range(37))
