@@ -724,24 +724,44 @@ __global__ __launch_bounds__(256) void les_mean_kernel(LesArgs<T> a, int ntx, in
   const bool col = i < g.mx && j < g.my;
   const int c0 = (ly + 1) * FBX + (lx + 1);  // slot of the column's node
   V Pm[2] = {V(0), V(0)}, P0[2] = {V(0), V(0)};
+  // plane kk's slots in registers: loaded one plane ahead, so that their
+  // loads are in flight during the sums of the plane before
+  constexpr int NSL = (FNB + 255) / 256;
+  V r0[NSL], r1[NSL];
+  auto fetch = [&](int kk) {
+#pragma unroll
+    for (int s = 0; s < NSL; s++) {
+      const int w = t + 256 * s;
+      r0[s] = V(0);
+      r1[s] = V(0);
+      const int ii = i0 - 1 + w % FBX, jj = j0 - 1 + w / FBX;
+      if (w >= FNB || kk < 0 || kk > g.mz - 1 || ii < 0 || jj < 0 || ii > g.mx - 1 || jj > g.my - 1) continue;
+      const long long m = nid(g, kk, jj, ii);
+      if (PART == 0) {
+        const bool in = interior(g, kk, jj, ii);
+        r0[s] = in ? a.lm[m] : T(0);  // LM is zero off the loop's nodes
+        r1[s] = in ? a.mm[m] : T(0);
+      } else {
+        r0[s] = a.ivol[m];  // 1. / aj
+        r1[s] = lcs_at(a, kk, jj, ii);
+      }
+    }
+  };
+  fetch(k0 - 1);
   for (int kk = k0 - 1; kk <= k1 + 1; kk++) {
     const bool st = kk >= 0 && kk <= g.mz - 1;  // uniform
     if (st) {
-      for (int w = t; w < FNB; w += 256) {
-        const int ii = i0 - 1 + w % FBX, jj = j0 - 1 + w / FBX;
-        if (ii < 0 || jj < 0 || ii > g.mx - 1 || jj > g.my - 1) continue;
-        const long long m = nid(g, kk, jj, ii);
-        if (PART == 0) {
-          const bool in = interior(g, kk, jj, ii);
-          B[0][w] = in ? a.lm[m] : T(0);  // LM is zero off the loop's nodes
-          B[1][w] = in ? a.mm[m] : T(0);
-        } else {
-          B[0][w] = a.ivol[m];  // 1. / aj
-          B[1][w] = lcs_at(a, kk, jj, ii);
+#pragma unroll
+      for (int s = 0; s < NSL; s++) {
+        const int w = t + 256 * s;
+        if (w < FNB) {
+          B[0][w] = r0[s];
+          B[1][w] = r1[s];
         }
       }
     }
     __syncthreads();
+    if (kk <= k1) fetch(kk + 1);
     V P1[2] = {V(0), V(0)};
     if (st && col && i >= 1 && j >= 1 && i <= g.mx - 2 && j <= g.my - 2) {
 #pragma unroll
