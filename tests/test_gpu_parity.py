@@ -613,14 +613,18 @@ def test_vtk_out_pressure_lines(tmp_path):
         assert g.vtk_out(str(tmp_path), block=0, pres=1)[1] == " PL \n "
 
 
-@pytest.mark.parametrize("case", ["tgv_solid", "channel"])
+@pytest.mark.parametrize("case", ["tgv_solid", "channel", "curvilinear_vortex"])
 def test_q_criteria(case):
     """QCriteria (data.c:4815-5053) on the device equals the oracle's
     restatement bitwise, from the same state, nvert and device metrics: a
     periodic Taylor-Green state with solid nodes (one-sided differences around
-    them; nvert 1 is not solid there) and a stepped channel on a stretched grid;
-    boundary nodes keep the pressure and a lambda2 of 0."""
-    if case == "tgv_solid":
+    them; nvert 1 is not solid there), a stepped channel on a stretched grid
+    and a vortex stepped on a curvilinear grid (every metric component
+    non-zero); boundary nodes keep the pressure and a lambda2 of 0."""
+    if case == "curvilinear_vortex":
+        cfg, coor, Q0 = small_vortex(N=12)
+        nv = None
+    elif case == "tgv_solid":
         cfg, coor, Q0 = cases.tgv_case(N=12)
         Q0 = perturbed(Q0, 8, amp=5e-2)
         nv = np.zeros(Q0.shape[:3])
