@@ -43,6 +43,8 @@ extern PetscInt i_homo_filter, j_homo_filter, k_homo_filter;
 
 PetscErrorCode PetscOptionsGetInt(void *options, const char *pre, const char *name, PetscInt *v, PetscBool *set);
 PetscErrorCode PetscOptionsGetReal(void *options, const char *pre, const char *name, PetscReal *v, PetscBool *set);
+PetscErrorCode PetscOptionsGetBool(void *options, const char *pre, const char *name, PetscBool *v, PetscBool *set);
+PetscErrorCode PetscPrintf(int comm, const char *format, ...);
 PetscErrorCode DMGetCoordinates(DM dm, Vec *c);
 PetscErrorCode VecGetArrayRead(Vec v, const PetscScalar **a);
 PetscErrorCode VecRestoreArrayRead(Vec v, const PetscScalar **a);
@@ -50,6 +52,7 @@ PetscErrorCode VecGetArray(Vec v, PetscScalar **a);
 PetscErrorCode VecRestoreArray(Vec v, PetscScalar **a);
 PetscErrorCode PetscError(int comm, int line, const char *func, const char *file, int n, int p, const char *mess, ...);
 #define PETSC_COMM_SELF 0
+#define PETSC_COMM_WORLD 0
 #define SETERRQ1(comm, n, s, a1) return PetscError(comm, __LINE__, __func__, __FILE__, n, 1, s, a1)
 
 #endif
