@@ -933,10 +933,10 @@ int launch_les_t(cles_ctx *c, hipStream_t stream, const double *const *qs_cell) 
       CLES_HIP(c, hipGetLastError());
     } else {
       const int mtx = (c->g.mx + FTX - 1) / FTX, mty = (c->g.my + FTY - 1) / FTY;
-      // 16-plane chunks (at 256^3: 16 planes 2.35 ms of LES model, 8 2.36, 32
-      // 2.43, the node tiles below 2.39), fewer where a launch would hold
-      // fewer than 1024 workgroups
-      int kc = 16;
+      // 8-plane chunks (at 256^3, each plane's loads issued during the sums
+      // of the plane before: 8 planes 2.19 ms of LES model, 16 2.21, 32
+      // 2.24), fewer where a launch would hold fewer than 1024 workgroups
+      int kc = 8;
       while (kc > 4 && mtx * mty * ((c->g.mz + kc - 1) / kc) < 1024) kc /= 2;
       const int nt = mtx * mty * ((c->g.mz + kc - 1) / kc), px = (nt + 7) / 8;
       hipLaunchKernelGGL((les_mean_kernel<T, 0>), dim3(8 * px), dim3(256), 0, stream, a, mtx, mty, kc, nt, px);
